@@ -1,0 +1,325 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched transient F-statistic map on one MI355X.
+
+Default workload: rectangular-window maps for 4096 templates with ragged
+N_sft ~ U[1000, 8000] (10% random gaps each, 5% with an injected transient),
+N_t0 = N_tau = N_sft // 4, lnBtSG and the posterior modes fused
+(BtSG=True) -- 6.3e9 2F cells per step.  One step = the whole batch through
+the device path (binned merge + compensated prefix, map kernels with fused
+reductions, per-template finalize), replayed from a hipGraph; the atoms are
+resident in HBM before the timed region.
+
+Prints ONE JSON line.  Besides the contract fields it carries:
+  roofline      the map kernel: 4 B per 2F cell written / its event-timed
+                duration, against MI355X HBM peak (8 TB/s)
+  plain         the same kernels launched once per template from a Python loop
+                with a sync per template (the structure of TransientGridSearch),
+                on a sample of templates, and the batched/plain speedup
+  cpu_baseline  the FP64 C oracle (OpenMP over templates) on a bounded sample
+  parity        max relative 2F error, argmax and lnBtSG agreement against the
+                oracle on a sample of templates
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+METRIC = ("2F(t0,τ) cells/s on 1 GPU, batched vs plain at ≤1e-4 rel err; "
+          "% HBM/MFMA roofline")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(args):
+    from pyfstat_amd import synth
+
+    if args.workload == "rect_batched":
+        return synth.workload_rect_batched(T=args.templates)
+    raise ValueError(args.workload)
+
+
+def plain_loop(b, batch, idx, BtSG):
+    """The same kernels, one template at a time, sync per template."""
+    import torch
+
+    from pyfstat_amd import _hiplib
+
+    lib = b.lib
+    dev = b.device
+    max_cells = int(max(b.F_off[t + 1] - b.F_off[t] for t in idx))
+    F = torch.empty(max_cells, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    ts_base = b.ts_d.data_ptr()
+    cols_base = b.cols_d.data_ptr()
+    flags = _hiplib.TCW_FLAG_BTSG if BtSG else 0
+    st = _hiplib.Stats()
+    at = _hiplib.Atoms()
+    cells = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in idx:
+        a, e = int(batch.off[t]), int(batch.off[t + 1])
+        at.numDet = 1
+        at.TAtom = batch.TAtom
+        at.ts = ts_base + 4 * a
+        at.cols = cols_base + 4 * a
+        at.col_stride = b.col_stride
+        at.det_off[0] = 0
+        at.det_off[1] = e - a
+        at.t_min = int(batch.ts[a])
+        at.t_max = int(batch.ts[e - 1])
+        win = _hiplib.window_struct((batch.windows[t].type, batch.windows[t].t0,
+                                     batch.windows[t].t0Band, batch.windows[t].dt0,
+                                     batch.windows[t].tau, batch.windows[t].tauBand,
+                                     batch.windows[t].dtau))
+        _hiplib.check(lib.tcw_map(b.handle, ctypes.byref(at), ctypes.byref(win),
+                                  ctypes.c_void_p(F.data_ptr()), ctypes.byref(st), flags,
+                                  ctypes.c_void_p(stream.cuda_stream)), "tcw_map")
+        cells += int(st.N_t0 * st.N_tau)
+    torch.cuda.synchronize()
+    return cells, time.perf_counter() - t0
+
+
+def cpu_baseline(batch, budget_s, seed=5):
+    """FP64 C oracle, OpenMP over templates, on a seeded sample of templates."""
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp:
+        cores = min(cores, int(omp))
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(batch.T)
+    merged, TAs, t0s, wins, cells = [], [], [], [], []
+    for t in order:
+        a, e = int(batch.off[t]), int(batch.off[t + 1])
+        m, t0d = oracle.merge_binned(batch.ts[a:e], batch.cols[:, a:e],
+                                     np.array([0, e - a]), batch.TAtom)
+        merged.append(m)
+        TAs.append(batch.TAtom)
+        t0s.append(t0d)
+        wins.append(batch.windows[t])
+        w = batch.windows[t]
+        cells.append((w.t0Band // w.dt0 + 1) * (w.tauBand // w.dtau + 1))
+        if len(merged) >= 4 * cores and len(merged) % cores == 0:
+            break
+    # calibrate on `cores` templates, then size the sample to the budget
+    n0 = cores
+    t0 = time.perf_counter()
+    oracle.compute_map_batch(merged[:n0], TAs[:n0], t0s[:n0], wins[:n0], with_lnBtSG=True)
+    dt = time.perf_counter() - t0
+    per_cell = dt / max(sum(cells[:n0]), 1)
+    want_cells = budget_s / per_cell
+    k = n0
+    while k < len(merged) and sum(cells[:k]) < want_cells:
+        k += 1
+    if k > n0:
+        t0 = time.perf_counter()
+        oracle.compute_map_batch(merged[:k], TAs[:k], t0s[:k], wins[:k], with_lnBtSG=True)
+        dt = time.perf_counter() - t0
+    else:
+        k = n0
+    ncells = sum(cells[:k])
+    return {
+        "value": ncells / dt,
+        "unit": "cells/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{k} seeded-random templates of the workload ({ncells:.3e} cells, "
+                  f"{dt:.1f} s), FP64 C oracle with lnBtSG, OpenMP over templates",
+    }
+
+
+def parity_sample(b, batch, n, seed=3):
+    import oracle
+
+    if n <= 0:
+        return None
+    rng = np.random.default_rng(seed)
+    idx = sorted(rng.choice(batch.T, size=min(n, batch.T), replace=False).tolist())
+    st = b.stats()
+    worst, worst_lnb, argmax_exact, near = 0.0, 0.0, 0, 0
+    for t in idx:
+        a, e = int(batch.off[t]), int(batch.off[t + 1])
+        m, t0d = oracle.merge_binned(batch.ts[a:e], batch.cols[:, a:e], np.array([0, e - a]),
+                                     batch.TAtom)
+        om = oracle.compute_map(m, batch.TAtom, t0d, batch.windows[t])
+        lnb = oracle.lnBtSG(om.F_mn, om.maxF)
+        F = b.F_d[int(b.F_off[t]):int(b.F_off[t + 1])].cpu().numpy().reshape(om.F_mn.shape)
+        mask = np.abs(om.cond / 1e4 - 1) < 1e-3
+        near += int(mask.sum())
+        rel = np.abs(F - om.F_mn) / np.maximum(np.abs(om.F_mn), 1e-30)
+        worst = max(worst, float(np.where(mask, 0, rel).max()))
+        worst_lnb = max(worst_lnb, abs(float(st[t]["lnBtSG"]) - lnb))
+        argmax_exact += int((int(st[t]["m_ML"]), int(st[t]["n_ML"])) == (om.m_max, om.n_max))
+    return {
+        "templates": len(idx),
+        "max_rel_err_2F": worst,
+        "argmax_exact": f"{argmax_exact}/{len(idx)}",
+        "max_abs_err_lnBtSG": worst_lnb,
+        "cells_near_cond_threshold": near,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="rect_batched", choices=["rect_batched"])
+    ap.add_argument("--templates", type=int, default=4096)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-btsg", action="store_true")
+    ap.add_argument("--plain-templates", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--check-templates", type=int, default=6)
+    ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
+                    metavar=("FETCH_KIB", "WRITE_KIB"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE of the map kernel (KiB per "
+                         "dispatch, already scaled) to report as roofline.traffic")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(templates are independent: a multi-GPU run would shard them by cost).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from pyfstat_amd import tcw_fstat_map_funcs as tcw
+    from pyfstat_amd.batched import BatchedTransientMaps
+
+    features, _ = tcw.init_transient_fstat_map_features("hip")
+    log("generating workload ...")
+    t_gen = time.perf_counter()
+    batch = make_workload(args)
+    log(f"workload ready in {time.perf_counter() - t_gen:.1f} s: T={batch.T}, "
+        f"atoms={batch.ts.size}, cells={int(batch.cells().sum()):.3e}")
+    BtSG = not args.no_btsg
+    use_graph = not args.no_graph
+    b = BatchedTransientMaps.from_synthetic(batch, BtSG=BtSG)
+    cells = b.total_cells
+    atoms_binned = int(batch.n_sft.sum())
+    log(f"plan: {cells} cells, scratch {b.scratch_bytes() / 2**30:.2f} GiB")
+
+    for _ in range(args.warmup):
+        b.run(use_graph=use_graph)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        b.run(use_graph=use_graph, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms_per_step = e0.elapsed_time(e1) / args.steps
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall:.3f} s)")
+
+    # per-kernel durations: HIP events around each stage on the launch stream
+    reps = max(3, min(args.steps, 10))
+    stage_ms = np.zeros(3)
+    for _ in range(reps):
+        stage_ms += np.array(b.run_timed(stream=stream))
+    stage_ms /= reps
+    log(f"stage/maps/finalize ms: {stage_ms.round(4).tolist()}")
+
+    parity = parity_sample(b, batch, args.check_templates)
+    if parity:
+        log(f"parity: {parity}")
+
+    plain = None
+    if args.plain_templates > 0:
+        rng = np.random.default_rng(9)
+        idx = sorted(rng.choice(batch.T, size=min(args.plain_templates, batch.T),
+                                replace=False).tolist())
+        plain_loop(b, batch, idx[:8], BtSG)  # warm the single-map path
+        pc, pt = plain_loop(b, batch, idx, BtSG)
+        plain_rate = pc / pt
+        plain = {
+            "value": plain_rate,
+            "unit": "cells/s",
+            "templates": len(idx),
+            "ms_per_template": 1e3 * pt / len(idx),
+            "batched_over_plain": (cells / (ms_per_step / 1e3)) / plain_rate,
+        }
+        log(f"plain: {plain}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(batch, args.cpu_seconds)
+        log(f"cpu: {cpu}")
+
+    value = cells / (ms_per_step / 1e3)
+    map_bytes = 4.0 * cells
+    achieved = map_bytes / (stage_ms[1] / 1e3) / 1e9
+    traffic = None
+    if args.traffic_kib:
+        traffic = (args.traffic_kib[0] + args.traffic_kib[1]) * 1024.0
+    job_bytes = 4.0 * cells + 28.0 * atoms_binned
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "rect_batched: 4096 templates, N_sft~U[1000,8000], 10% gaps, "
+                        "N_t0=N_tau=N_sft//4, rect window, BtSG",
+            "templates": batch.T,
+            "cells": cells,
+            "binned_atoms": atoms_binned,
+            "BtSG": BtSG,
+            "hipgraph": use_graph,
+            "arithmetic": "fp32 two-sum prefix scan, fp64 window differences/determinant/"
+                          "normalisation, fp32 map",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "rect_fast_kernel (all map work items of the step)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": map_bytes,
+            "launch_ms": stage_ms[1],
+            "stage_ms": stage_ms[0],
+            "finalize_ms": stage_ms[2],
+            "job_GBps": job_bytes / (ms_per_step / 1e3) / 1e9,
+        },
+        "cpu_baseline": cpu,
+        "plain": plain,
+        "parity": parity,
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
+if __name__ == "__main__":
+    main()

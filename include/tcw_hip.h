@@ -1,0 +1,181 @@
+/*
+ * tcw_hip.h -- C ABI of the MI355X (gfx950) transient-CW F-statistic map library
+ * (libtcw_hip.so).  Plain C types and pointers only; device buffers are raw HIP
+ * device pointers owned by the caller, streams are hipStream_t passed as void*.
+ *
+ * The library replaces, one for one, the device-side work of PyFstat's
+ * tCWFstatMapVersion="pycuda" backend:
+ *   pyfstat/tcw_fstat_map_funcs.py:656-834  pycuda_compute_transient_fstat_map
+ *   pyfstat/tcw_fstat_map_funcs.py:837-916  ..._rect (H2D, kernel, D2H)
+ *   pyfstat/tcw_fstat_map_funcs.py:919-998  ..._exp
+ *   pyfstat/pyCUDAkernels/cudaTransientFstatRectWindow.cu:1-130
+ *   pyfstat/pyCUDAkernels/cudaTransientFstatExpWindow.cu:1-138
+ * plus the host-side steps the reference runs around them on the CPU:
+ *   lalpulsar.mergeMultiFstatAtomsBinned (called at tcw_fstat_map_funcs.py:706),
+ *   max/argmax (tcw_fstat_map_funcs.py:809-815),
+ *   get_lnBtSG / get_t0_max_posterior / get_tau_max_posterior
+ *   (tcw_fstat_map_funcs.py:196-287, called at 824-828).
+ *
+ * Error convention: every entry point returns 0 (TCW_OK) or a TCW_ERR_* code;
+ * tcw_last_error() then gives a message (thread-local, valid until the next
+ * call on the same thread).
+ *
+ * Threading: one handle per host thread / stream.  Calls on one handle are not
+ * re-entrant; distinct handles are independent.
+ */
+#ifndef TCW_HIP_H
+#define TCW_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCW_NCOL 7          /* a2, b2, ab, Fa_re, Fa_im, Fb_re, Fb_im */
+#define TCW_MAX_DETECTORS 10 /* lalpulsar PULSAR_MAX_DETECTORS */
+
+/* lalpulsar transientWindowType_t (pyfstat/core.py:829-833) */
+#define TCW_TRANSIENT_NONE 0u
+#define TCW_TRANSIENT_RECTANGULAR 1u
+#define TCW_TRANSIENT_EXPONENTIAL 2u
+#define TCW_TRANSIENT_LAST 3u
+
+/* return codes */
+#define TCW_OK 0
+#define TCW_ERR_INVALID 1     /* bad argument (maps to ValueError)            */
+#define TCW_ERR_DEVICE 2      /* HIP runtime failure (maps to RuntimeError)   */
+#define TCW_ERR_SINGLE_ATOM 3 /* lal-strict mode: a window spans one atom     */
+#define TCW_ERR_NOMEM 4       /* device allocation failed                     */
+
+/* flags */
+#define TCW_FLAG_BTSG 1u       /* also lnBtSG and the (t0, tau) posterior modes */
+#define TCW_FLAG_LAL_STRICT 2u /* fail like lalpulsar (XLAL_EDOM) on i_t1==i_t0 */
+#define TCW_FLAG_GENERIC 4u    /* force the generic per-cell kernels (testing)  */
+
+/* transientWindowRange_t: UINT4 fields (pyfstat/core.py:1447 "TYPE UINT4") */
+typedef struct {
+    uint32_t type, t0, t0Band, dt0, tau, tauBand, dtau;
+} tcw_window;
+
+/* One template's atoms, detector-concatenated, already on the device.
+ * ts[det_off[X] .. det_off[X+1]) are detector X's timestamps (ascending);
+ * cols[c*col_stride + k] is column c of atom k.  det_off, t_min, t_max live on
+ * the host (t_min/t_max: smallest first / largest last timestamp over all
+ * detectors, which fix the binned length). */
+typedef struct {
+    uint32_t numDet;
+    uint32_t TAtom;
+    const uint32_t *ts;   /* device */
+    const float *cols;    /* device, [TCW_NCOL][col_stride] */
+    int64_t col_stride;
+    int64_t det_off[TCW_MAX_DETECTORS + 1];
+    uint32_t t_min, t_max;
+} tcw_atoms;
+
+/* Per-map results (FstatMap fields, tcw_fstat_map_funcs.py:134-144). */
+typedef struct {
+    float maxF;              /* F, not 2F */
+    int32_t single_atom;     /* 1 if any cell has i_t1 == i_t0 */
+    int64_t m_ML, n_ML;      /* first row-major argmax */
+    int64_t N_t0, N_tau;
+    int64_t t0_ML, tau_ML;   /* effective window t0 + m*dt0, tau + n*dtau */
+    double lnBtSG;           /* NaN unless TCW_FLAG_BTSG */
+    int64_t m_MP, n_MP;      /* posterior-mode bins, -1 unless TCW_FLAG_BTSG */
+    double t0_MP, tau_MP;    /* caller's window: t0 + (m_MP+0.5)*t0Band/N_t0 ... */
+    uint32_t numAtoms;       /* binned atoms */
+    uint32_t t0_data;
+} tcw_stats;
+
+typedef struct tcw_handle tcw_handle;
+
+/* Library version string. */
+const char *tcw_version(void);
+
+/* Thread-local message of the last failing call on this thread. */
+const char *tcw_last_error(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int tcw_device_count(void);
+
+/* Marketing name of device `device` (hipDeviceProp_t::name) into buf. */
+int tcw_device_name(int device, char *buf, int buflen);
+
+/* Map shape after the TRANSIENT_NONE conversion (tcw_fstat_map_funcs.py:742-780):
+ * N_t0 = floor(t0Band/dt0)+1, N_tau = floor(tauBand/dtau)+1. */
+int tcw_map_shape(uint32_t numAtoms, uint32_t TAtom, uint32_t t0_data,
+                  const tcw_window *win, int64_t *N_t0, int64_t *N_tau);
+
+/* Handle owning device scratch (merged atoms, prefix sums, reduction partials)
+ * on `device`; replaces pycuda's context (tcw_fstat_map_funcs.py:361-492). */
+int tcw_create(int device, tcw_handle **out);
+int tcw_destroy(tcw_handle *h);
+
+/* One transient F-statistic map: merge + window sums + F + fused reductions.
+ * Replaces pycuda_compute_transient_fstat_map (tcw_fstat_map_funcs.py:656-834).
+ * F_mn_dev: device float32 [N_t0][N_tau] (F, not 2F), caller-allocated.
+ * stats: host struct, filled before return (the call synchronises `stream`). */
+int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win,
+            float *F_mn_dev, tcw_stats *stats, uint32_t flags, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched maps: many templates (ragged atom counts and windows) in one pass.
+ * Replaces the per-template Python loop of TransientGridSearch.run
+ * (pyfstat/grid_based_searches.py:1186-1189) over get_transient_detstats
+ * (pyfstat/core.py:1412-1471).
+ *
+ * Template t owns detector segments seg_begin[t] .. seg_begin[t+1]-1; segment
+ * s owns atoms seg_off[s] .. seg_off[s+1]-1 of ts/cols.  All host arrays are
+ * read during tcw_batch_plan only.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int64_t T;                 /* templates */
+    uint32_t TAtom;            /* common atom length */
+    const int64_t *seg_begin;  /* host [T+1] */
+    const int64_t *seg_off;    /* host [S+1] */
+    const uint32_t *seg_tfirst;/* host [S]: first timestamp of each segment */
+    const uint32_t *seg_tlast; /* host [S]: last timestamp of each segment */
+    const tcw_window *windows; /* host [T] */
+} tcw_batch_desc;
+
+typedef struct tcw_plan tcw_plan;
+
+/* Plan a batch: shapes, output offsets, work items (cost-sorted), scratch.
+ * F_off (host, [T+1], may be NULL) receives each map's float offset into the
+ * concatenated output; the total float count is F_off[T]. */
+int tcw_batch_plan(tcw_handle *h, const tcw_batch_desc *desc, uint32_t flags,
+                   tcw_plan **out, int64_t *F_off);
+
+/* Device bytes of scratch the plan holds (prefix sums, partials, tables). */
+int64_t tcw_plan_scratch_bytes(const tcw_plan *p);
+
+/* Run a planned batch on device-resident atoms.  ts/cols follow the plan's
+ * segment layout (cols is [TCW_NCOL][col_stride]); F_out is the concatenated
+ * float32 output (F_off layout); stats_dev is a device array of T tcw_stats.
+ * Asynchronous on `stream`.  With use_graph != 0 the launch sequence is
+ * captured into a hipGraph on the first call and replayed afterwards while
+ * the pointers stay the same. */
+int tcw_batch_run(tcw_handle *h, tcw_plan *p, const uint32_t *ts,
+                  const float *cols, int64_t col_stride, float *F_out,
+                  tcw_stats *stats_dev, int use_graph, void *stream);
+
+/* tcw_batch_run without a graph, with HIP events recorded on `stream` around
+ * the stage kernel, the map kernels and the finalize kernel; waits for them
+ * and returns the three elapsed times in ms_out[0..2] (milliseconds).  For
+ * benchmarks and profiling; no counterpart in the reference. */
+int tcw_batch_run_timed(tcw_handle *h, tcw_plan *p, const uint32_t *ts,
+                        const float *cols, int64_t col_stride, float *F_out,
+                        tcw_stats *stats_dev, double *ms_out, void *stream);
+
+int tcw_plan_destroy(tcw_plan *p);
+
+/* Tuning hook used by the autotuner: rows of (t0) per work item for kernel
+ * mode 0 (rect, regular grid), 1 (rect, generic) or 2 (exp); a multiple of
+ * 16 in [16, 512], or 0 for the built-in default.  Affects later plans. */
+int tcw_set_rows_per_item(int mode, int rows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TCW_HIP_H */

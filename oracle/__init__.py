@@ -1,0 +1,269 @@
+"""FP64 CPU oracle of the transient F-statistic map -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import
+this package, and only as the checker.  The product (``pyfstat_amd``) never
+imports, links or runs anything here.
+
+Two independent restatements of the reference algorithm live here:
+
+* ``tcw_oracle.c`` (built by ``oracle/Makefile`` into ``oracle/build/``), bound
+  below through ctypes: the FP64 oracle used for parity at real sizes and as
+  the CPU baseline.  It follows lalpulsar's XLALComputeTransientFstatMap as the
+  reference mirrors it (pyfstat/tcw_fstat_map_funcs.py:656-834,
+  pyfstat/pyCUDAkernels/cudaTransientFstatRectWindow.cu:14-124,
+  pyfstat/pyCUDAkernels/cudaTransientFstatExpWindow.cu:15-134).
+* ``twin.py``: a plain-Python loop restatement of the same steps for tiny
+  inputs, used to cross-check the C oracle.
+
+How each function is pinned (closed forms, invariants, brute force, library
+routines) is listed in tests/test_oracle.py.  Parity unpinned: lalpulsar's
+lookup-table exp (XLALFastNegExp) and its REAL4 accumulation order, which only
+an import of lalpulsar could pin; the oracle uses exact exp and FP64 sums.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libtcw_oracle.so")
+
+TRANSIENT_NONE = 0
+TRANSIENT_RECTANGULAR = 1
+TRANSIENT_EXPONENTIAL = 2
+TRANSIENT_LAST = 3
+
+
+class _Window(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("type", "t0", "t0Band", "dt0", "tau", "tauBand", "dtau")]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("maxF", ctypes.c_double),
+        ("m_max", ctypes.c_int64),
+        ("n_max", ctypes.c_int64),
+        ("N_t0", ctypes.c_int64),
+        ("N_tau", ctypes.c_int64),
+        ("t0", ctypes.c_uint32),
+        ("dt0", ctypes.c_uint32),
+        ("tau", ctypes.c_uint32),
+        ("dtau", ctypes.c_uint32),
+        ("bad_m", ctypes.c_int64),
+        ("bad_n", ctypes.c_int64),
+    ]
+
+
+def build():
+    """Compile tcw_oracle.c with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER
+        lib.orc_merge_size.restype = ctypes.c_int64
+        lib.orc_merge_size.argtypes = [ctypes.c_int, P(ctypes.c_int64),
+                                       P(ctypes.c_uint32), ctypes.c_uint32,
+                                       P(ctypes.c_uint32)]
+        lib.orc_merge.restype = ctypes.c_int
+        lib.orc_merge.argtypes = [ctypes.c_int, P(ctypes.c_int64), P(ctypes.c_uint32),
+                                  P(ctypes.c_float), ctypes.c_int64, ctypes.c_uint32,
+                                  ctypes.c_uint32, ctypes.c_int64, P(ctypes.c_float)]
+        lib.orc_map_shape.restype = ctypes.c_int
+        lib.orc_map_shape.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                      P(_Window), P(_Window), P(ctypes.c_int64),
+                                      P(ctypes.c_int64)]
+        lib.orc_map.restype = ctypes.c_int
+        lib.orc_map.argtypes = [P(ctypes.c_float), ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32, P(_Window), P(ctypes.c_double),
+                                P(ctypes.c_double), P(_Result)]
+        lib.orc_lnBtSG.restype = ctypes.c_double
+        lib.orc_lnBtSG.argtypes = [P(ctypes.c_double), ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_double]
+        lib.orc_posterior_modes.restype = None
+        lib.orc_posterior_modes.argtypes = [P(ctypes.c_double), ctypes.c_int64,
+                                            ctypes.c_int64, ctypes.c_double, P(_Window),
+                                            P(ctypes.c_double), P(ctypes.c_double)]
+        lib.orc_map_batch.restype = ctypes.c_int
+        lib.orc_map_batch.argtypes = [ctypes.c_int64, P(ctypes.c_float), P(ctypes.c_int64),
+                                      P(ctypes.c_uint32), P(ctypes.c_uint32),
+                                      P(ctypes.c_uint32), P(_Window), P(ctypes.c_double),
+                                      P(ctypes.c_int64), P(ctypes.c_double),
+                                      P(ctypes.c_int64), P(ctypes.c_int64),
+                                      P(ctypes.c_double)]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _win(w):
+    if isinstance(w, (tuple, list)):
+        return _Window(*[int(x) for x in w])
+    return _Window(*[int(getattr(w, f)) for f in
+                     ("type", "t0", "t0Band", "dt0", "tau", "tauBand", "dtau")])
+
+
+def merge_binned(ts, cols, det_off, TAtom):
+    """lalpulsar mergeMultiFstatAtomsBinned restated: float32 [7, N], t0_data.
+
+    ts uint32 [total], cols float32 [7, total], det_off int64 [numDet+1].
+    """
+    ts = np.ascontiguousarray(ts, dtype=np.uint32)
+    cols = np.ascontiguousarray(cols, dtype=np.float32)
+    det_off = np.ascontiguousarray(det_off, dtype=np.int64)
+    lib = _load()
+    tmin = ctypes.c_uint32(0)
+    n = lib.orc_merge_size(len(det_off) - 1, _ptr(det_off, ctypes.c_int64),
+                           _ptr(ts, ctypes.c_uint32), int(TAtom), ctypes.byref(tmin))
+    if n <= 0:
+        raise ValueError("bad atoms for merge")
+    out = np.empty((7, n), dtype=np.float32)
+    rc = lib.orc_merge(len(det_off) - 1, _ptr(det_off, ctypes.c_int64),
+                       _ptr(ts, ctypes.c_uint32), _ptr(cols, ctypes.c_float),
+                       cols.shape[1], int(TAtom), tmin.value, n,
+                       _ptr(out, ctypes.c_float))
+    if rc:
+        raise ValueError("merge failed")
+    return out, int(tmin.value)
+
+
+class OracleMap:
+    """FP64 map result: F (float64 [N_t0, N_tau]), maxF, argmax, effective window."""
+
+    def __init__(self, F, cond, res, win_eff):
+        self.F_mn = F
+        self.cond = cond
+        self.maxF = float(res.maxF)
+        self.m_max = int(res.m_max)
+        self.n_max = int(res.n_max)
+        self.t0_ML = int(res.t0) + self.m_max * int(res.dt0)
+        self.tau_ML = int(res.tau) + self.n_max * int(res.dtau)
+        self.bad_cell = None if res.bad_m < 0 else (int(res.bad_m), int(res.bad_n))
+        self.window_eff = win_eff
+
+
+def compute_map(merged, TAtom, t0_data, window):
+    """FP64 map over merged atoms (float32 [7, N]) for a window range."""
+    merged = np.ascontiguousarray(merged, dtype=np.float32)
+    lib = _load()
+    w = _win(window)
+    eff = _Window()
+    nt0 = ctypes.c_int64()
+    ntau = ctypes.c_int64()
+    N = merged.shape[1]
+    if lib.orc_map_shape(N, int(TAtom), int(t0_data), ctypes.byref(w), ctypes.byref(eff),
+                         ctypes.byref(nt0), ctypes.byref(ntau)):
+        raise ValueError(f"bad window {window}")
+    F = np.empty((nt0.value, ntau.value), dtype=np.float64)
+    cond = np.empty_like(F)
+    res = _Result()
+    rc = lib.orc_map(_ptr(merged, ctypes.c_float), N, int(TAtom), int(t0_data),
+                     ctypes.byref(w), _ptr(F, ctypes.c_double), _ptr(cond, ctypes.c_double),
+                     ctypes.byref(res))
+    if rc:
+        raise ValueError("oracle map failed")
+    win_eff = tuple(int(getattr(eff, f)) for f in
+                    ("type", "t0", "t0Band", "dt0", "tau", "tauBand", "dtau"))
+    return OracleMap(F, cond, res, win_eff)
+
+
+def lnBtSG(F, maxF=None):
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    if maxF is None:
+        maxF = float(F.max())
+    return float(_load().orc_lnBtSG(_ptr(F, ctypes.c_double), F.shape[0], F.shape[1],
+                                    float(maxF)))
+
+
+def posterior_modes(F, window, maxF=None):
+    """(t0_MP, tau_MP) from the caller's window range (tcw.py:219-287)."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    if maxF is None:
+        maxF = float(F.max())
+    w = _win(window)
+    t0 = ctypes.c_double()
+    tau = ctypes.c_double()
+    _load().orc_posterior_modes(_ptr(F, ctypes.c_double), F.shape[0], F.shape[1],
+                                float(maxF), ctypes.byref(w), ctypes.byref(t0),
+                                ctypes.byref(tau))
+    return t0.value, tau.value
+
+
+def compute_transient_fstat_map(packed, window, BtSG=False):
+    """Full oracle path from packed per-detector atoms: merge, map, stats.
+
+    ``packed`` has ``ts``, ``cols``, ``det_off``, ``TAtom``
+    (pyfstat_amd.atoms.PackedAtoms or any object with those fields).
+    Returns an OracleMap with lnBtSG/t0_MP/tau_MP set when BtSG.
+    """
+    merged, t0_data = merge_binned(packed.ts, packed.cols, packed.det_off, packed.TAtom)
+    om = compute_map(merged, packed.TAtom, t0_data, window)
+    om.merged = merged
+    om.t0_data = t0_data
+    om.lnBtSG = float("nan")
+    om.t0_MP = float("nan")
+    om.tau_MP = float("nan")
+    if BtSG:
+        om.lnBtSG = lnBtSG(om.F_mn, om.maxF)
+        om.t0_MP, om.tau_MP = posterior_modes(om.F_mn, window, om.maxF)
+    return om
+
+
+def compute_map_batch(merged_list, TAtoms, t0_datas, windows, with_lnBtSG=False):
+    """Many maps at once, OpenMP over templates (CPU baseline).
+
+    Returns (list of F arrays, maxF [T], m_max [T], n_max [T], lnBtSG [T] or None).
+    """
+    lib = _load()
+    T = len(merged_list)
+    sizes = np.array([m.shape[1] for m in merged_list], dtype=np.uint32)
+    atoms_off = np.zeros(T + 1, dtype=np.int64)
+    atoms_off[1:] = np.cumsum(7 * sizes.astype(np.int64))
+    atoms = np.empty(int(atoms_off[-1]), dtype=np.float32)
+    shapes = []
+    wins = (_Window * T)()
+    for t, m in enumerate(merged_list):
+        atoms[atoms_off[t]:atoms_off[t + 1]] = np.ascontiguousarray(m, np.float32).ravel()
+        w = _win(windows[t])
+        wins[t] = w
+        eff = _Window()
+        nt0 = ctypes.c_int64()
+        ntau = ctypes.c_int64()
+        if lib.orc_map_shape(int(sizes[t]), int(TAtoms[t]), int(t0_datas[t]), ctypes.byref(w),
+                             ctypes.byref(eff), ctypes.byref(nt0), ctypes.byref(ntau)):
+            raise ValueError("bad window")
+        shapes.append((nt0.value, ntau.value))
+    F_off = np.zeros(T + 1, dtype=np.int64)
+    F_off[1:] = np.cumsum([a * b for a, b in shapes])
+    F = np.empty(int(F_off[-1]), dtype=np.float64)
+    maxF = np.empty(T)
+    m_max = np.empty(T, dtype=np.int64)
+    n_max = np.empty(T, dtype=np.int64)
+    lnB = np.empty(T) if with_lnBtSG else None
+    TA = np.ascontiguousarray(TAtoms, dtype=np.uint32)
+    T0 = np.ascontiguousarray(t0_datas, dtype=np.uint32)
+    rc = lib.orc_map_batch(
+        T, _ptr(atoms, ctypes.c_float), _ptr(atoms_off, ctypes.c_int64),
+        _ptr(sizes, ctypes.c_uint32), _ptr(TA, ctypes.c_uint32), _ptr(T0, ctypes.c_uint32),
+        wins,
+        _ptr(F, ctypes.c_double), _ptr(F_off, ctypes.c_int64), _ptr(maxF, ctypes.c_double),
+        _ptr(m_max, ctypes.c_int64), _ptr(n_max, ctypes.c_int64),
+        _ptr(lnB, ctypes.c_double) if lnB is not None else None)
+    if rc:
+        raise ValueError("oracle batch failed")
+    Fs = [F[F_off[t]:F_off[t + 1]].reshape(shapes[t]) for t in range(T)]
+    return Fs, maxF, m_max, n_max, lnB

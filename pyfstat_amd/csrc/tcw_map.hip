@@ -1,0 +1,1453 @@
+// tcw_map.hip -- transient-CW F-statistic maps on MI355X (gfx950).
+//
+// Device path for one or many templates:
+//   1. stage_kernel    binned detector merge (lalpulsar mergeMultiFstatAtomsBinned,
+//                      called at pyfstat/tcw_fstat_map_funcs.py:706) fused with a
+//                      compensated FP32 (two-sum) prefix scan of the 7 atom
+//                      columns (rect windows) or an FP64 atom table (exp windows);
+//   2. map kernels     window sums -> cond-guarded F per (t0, tau) cell, with the
+//                      max/argmax, logsumexp and row/column posterior sums fused
+//                      into the producer (pyfstat/tcw_fstat_map_funcs.py:809-828);
+//   3. finalize_kernel per-template combination of the partials into FstatMap
+//                      fields (maxF, t0_ML, tau_ML, lnBtSG, t0_MP, tau_MP).
+// Host side: the C ABI declared in include/tcw_hip.h.
+//
+// Index maths and window semantics follow the reference kernels
+// pyfstat/pyCUDAkernels/cudaTransientFstatRectWindow.cu:21-124 and
+// pyfstat/pyCUDAkernels/cudaTransientFstatExpWindow.cu:27-134 (uint32
+// wrap-around, clamping, incremental rect sums, cond < 1e4 guard, F = 2
+// fallback); the arithmetic is new (prefix differences, FP64 determinant).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tcw_hip.h"
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// constants and device-side descriptors
+// ----------------------------------------------------------------------------
+
+constexpr int NC = TCW_NCOL;
+constexpr int WG = 256;           // threads per workgroup (4 wave64)
+constexpr int TILE_COLS = 256;    // tau columns per work item
+constexpr int MERGED_STRIDE = 8;  // doubles per atom in the FP64 atom table
+constexpr double COND_MAX = 1e4;  // rect.cu:104
+// cond < K  <=>  T^2 < Q*D  with T = A+B, D = AB-C^2 > 0, Q = (K+1)^2/K, because
+// cond = (T+s)/(T-s), s^2 = T^2-4D, and (1+cond)^2/cond = T^2/D grows with cond.
+constexpr double QCOND = (COND_MAX + 1.0) * (COND_MAX + 1.0) / COND_MAX;
+
+enum Mode : int32_t {
+    MODE_RECT_FAST = 0,     // regular grid: dt0 == dtau == a*TAtom, no wrap-around
+    MODE_RECT_GENERIC = 1,  // any rectangular window range, exact uint32 semantics
+    MODE_EXP_DIRECT = 2,    // exponential window, direct weighted sum
+    MODE_COUNT = 3
+};
+
+constexpr int RB_DEFAULT[MODE_COUNT] = {64, 16, 16};
+
+struct TmplDev {
+    int64_t seg_begin, seg_end;  // detector segments of this template
+    int64_t merged_off;          // doubles, FP64 atom table [N][8]   (exp modes)
+    int64_t prefix_off;          // doubles, prefix entries [N+1][7]  (rect modes)
+    int64_t F_off;               // floats, map [N_t0][N_tau]
+    int64_t item_begin;          // first work item of this template
+    int64_t rowpart_off;         // float2, [N_t0][n_colblk]
+    int64_t colpart_off;         // float2, [N_tau][n_rowblk]
+    uint32_t N, TAtom, t_min;    // binned atoms, atom length, t0_data
+    uint32_t type, t0, dt0, tau, dtau;  // effective window (after NONE -> RECT)
+    uint32_t N_t0, N_tau;
+    uint32_t n_rowblk, n_colblk;
+    int32_t mode;
+    int32_t rb_rows;             // rows per work item
+    int32_t q0, q00, a;          // MODE_RECT_FAST index maths
+    uint32_t o_t0, o_t0Band, o_tau, o_tauBand;  // caller's window (posterior modes)
+    int32_t pad0, pad1;
+};
+
+struct SegDev {
+    int64_t begin, end;
+};
+
+// the Python binding (pyfstat_amd/_hiplib.py) mirrors these layouts
+static_assert(sizeof(tcw_stats) == 104, "tcw_stats layout");
+static_assert(sizeof(tcw_atoms) == 128, "tcw_atoms layout");
+static_assert(sizeof(tcw_window) == 28, "tcw_window layout");
+
+struct ItemDev {
+    int32_t t, rb, cb, pad;
+};
+
+struct ItemPart {
+    float maxF;
+    int32_t idx;
+    float M, S;  // sum of exp(F - M) over the item's cells
+};
+
+// ----------------------------------------------------------------------------
+// FP32 error-free transformations (two-sum) for the compensated prefix scan
+// ----------------------------------------------------------------------------
+
+struct df {
+    float hi, lo;
+};
+
+__device__ __forceinline__ df two_sum(float a, float b) {
+    float s = a + b;
+    float bb = s - a;
+    float e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+
+__device__ __forceinline__ df fast_two_sum(float a, float b) {
+    float s = a + b;
+    float e = b - (s - a);
+    return {s, e};
+}
+
+__device__ __forceinline__ df df_add(df a, df b) {
+    df s = two_sum(a.hi, b.hi);
+    df t = two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = fast_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return fast_two_sum(s.hi, s.lo);
+}
+
+__device__ __forceinline__ df df_add_f(df a, float b) {
+    df s = two_sum(a.hi, b);
+    s.lo += a.lo;
+    return fast_two_sum(s.hi, s.lo);
+}
+
+// exact value of a (hi, lo) pair: 2 x 24 bits fit a double
+__device__ __forceinline__ double df_value(df a) { return (double)a.hi + (double)a.lo; }
+
+// ----------------------------------------------------------------------------
+// stage: binned merge + compensated prefix (rect) / FP64 atom table (exp)
+// ----------------------------------------------------------------------------
+
+// One workgroup per template.  Thread k owns a contiguous run of bins.  Each
+// bin's merged value is the float32 sum over detectors X = 0,1,... (in that
+// order) of the detector's atoms in the bin (in atom order), starting from 0 --
+// the same additions, in the same order, as lalpulsar's binned merge, so the
+// merged atoms are bit-identical.
+__global__ __launch_bounds__(WG) void stage_kernel(
+    const TmplDev *__restrict__ tds, const SegDev *__restrict__ segs,
+    const uint32_t *__restrict__ ts, const float *__restrict__ cols, int64_t cstride,
+    double *__restrict__ prefix, double *__restrict__ merged) {
+    const TmplDev td = tds[blockIdx.x];
+    const bool want_prefix = td.mode != MODE_EXP_DIRECT;
+    const uint32_t N = td.N, TA = td.TAtom;
+    const int tid = threadIdx.x;
+    const uint32_t chunk = (N + WG - 1) / WG;
+    const uint32_t b0 = min(N, (uint32_t)tid * chunk);
+    const uint32_t b1 = min(N, b0 + chunk);
+    const int nseg = (int)(td.seg_end - td.seg_begin);
+
+    int64_t cur0[TCW_MAX_DETECTORS], cur[TCW_MAX_DETECTORS], endk[TCW_MAX_DETECTORS];
+    const uint64_t tstart = (uint64_t)td.t_min + (uint64_t)b0 * TA;
+#pragma unroll
+    for (int X = 0; X < TCW_MAX_DETECTORS; X++) {
+        cur0[X] = 0;
+        endk[X] = 0;
+        if (X < nseg) {
+            const SegDev sg = segs[td.seg_begin + X];
+            int64_t lo = sg.begin, hi = sg.end;  // first atom with ts >= tstart
+            while (lo < hi) {
+                int64_t mid = (lo + hi) >> 1;
+                if ((uint64_t)ts[mid] < tstart)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            cur0[X] = lo;
+            endk[X] = sg.end;
+        }
+    }
+
+    auto merge_bin = [&](uint32_t b, float v[NC]) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = 0.0f;
+        const uint64_t lim = (uint64_t)td.t_min + (uint64_t)(b + 1) * TA;
+#pragma unroll
+        for (int X = 0; X < TCW_MAX_DETECTORS; X++) {
+            if (X < nseg) {
+                while (cur[X] < endk[X] && (uint64_t)ts[cur[X]] < lim) {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) v[c] += cols[c * cstride + cur[X]];
+                    cur[X]++;
+                }
+            }
+        }
+    };
+
+#pragma unroll
+    for (int X = 0; X < TCW_MAX_DETECTORS; X++) cur[X] = cur0[X];
+
+    if (!want_prefix) {
+        double *out = merged + td.merged_off;
+        for (uint32_t b = b0; b < b1; b++) {
+            float v[NC];
+            merge_bin(b, v);
+#pragma unroll
+            for (int c = 0; c < NC; c++) out[(int64_t)b * MERGED_STRIDE + c] = (double)v[c];
+            out[(int64_t)b * MERGED_STRIDE + NC] = 0.0;
+        }
+        return;
+    }
+
+    // pass 1: compensated totals of this thread's bins
+    df tot[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) tot[c] = {0.0f, 0.0f};
+    for (uint32_t b = b0; b < b1; b++) {
+        float v[NC];
+        merge_bin(b, v);
+#pragma unroll
+        for (int c = 0; c < NC; c++) tot[c] = df_add_f(tot[c], v[c]);
+    }
+
+    // exclusive scan of the 256 thread totals, one column per thread
+    __shared__ float sh_hi[NC][WG];
+    __shared__ float sh_lo[NC][WG];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        sh_hi[c][tid] = tot[c].hi;
+        sh_lo[c][tid] = tot[c].lo;
+    }
+    __syncthreads();
+    if (tid < NC) {
+        df run = {0.0f, 0.0f};
+        for (int k = 0; k < WG; k++) {
+            df x = {sh_hi[tid][k], sh_lo[tid][k]};
+            sh_hi[tid][k] = run.hi;
+            sh_lo[tid][k] = run.lo;
+            run = df_add(run, x);
+        }
+    }
+    __syncthreads();
+
+    // pass 2: re-merge and write prefix entries P[b] = sum of bins < b
+    df run[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) run[c] = {sh_hi[c][tid], sh_lo[c][tid]};
+#pragma unroll
+    for (int X = 0; X < TCW_MAX_DETECTORS; X++) cur[X] = cur0[X];
+    double *P = prefix + td.prefix_off;
+    for (uint32_t b = b0; b < b1; b++) {
+        float v[NC];
+        merge_bin(b, v);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            P[(int64_t)b * NC + c] = df_value(run[c]);
+            run[c] = df_add_f(run[c], v[c]);
+        }
+    }
+    if (b1 == N && b0 < b1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) P[(int64_t)N * NC + c] = df_value(run[c]);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// per-cell F
+// ----------------------------------------------------------------------------
+
+// F from window sums: determinant, condition guard and normalisation in FP64
+// (rect.cu:98-118 / exp.cu:109-129; F = 2 when the guard fails).
+__device__ __forceinline__ float cell_F(double A, double B, double C, double far,
+                                        double fai, double fbr, double fbi) {
+    const double D = fma(A, B, -(C * C));
+    const double T = A + B;
+    const double p = fma(far, far, fai * fai);
+    const double q = fma(fbr, fbr, fbi * fbi);
+    const double r = fma(far, fbr, fai * fbi);
+    const double num = fma(B, p, fma(A, q, -2.0 * C * r));
+    const bool ok = (T > 0.0) && (T * T < QCOND * D);
+    return ok ? (float)(num / D) : 2.0f;
+}
+
+// uint32 index maths of rect.cu:21-31 (start) and rect.cu:59-69 (end).
+__device__ __forceinline__ uint32_t idx_start(uint32_t t, uint32_t t_min, uint32_t TA,
+                                              uint32_t N) {
+    int32_t i = (int32_t)((uint32_t)(t - t_min + TA / 2) / TA);
+    if (i < 0) i = 0;
+    uint32_t u = (uint32_t)i;
+    return u >= N ? N - 1 : u;
+}
+
+__device__ __forceinline__ uint32_t idx_end_q(uint32_t q, uint32_t N) {
+    int32_t i = (int32_t)(q - 1u);
+    if (i < 0) i = 0;
+    uint32_t u = (uint32_t)i;
+    return u >= N ? N - 1 : u;
+}
+
+// ----------------------------------------------------------------------------
+// fused reductions
+// ----------------------------------------------------------------------------
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ void argmax_merge(float &bv, int &bi, float v, int i) {
+    if (v > bv || (v == bv && i < bi)) {
+        bv = v;
+        bi = i;
+    }
+}
+
+__device__ __forceinline__ void wave_argmax(float &bv, int &bi) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float ov = __shfl_xor(bv, o);
+        int oi = __shfl_xor(bi, o);
+        argmax_merge(bv, bi, ov, oi);
+    }
+}
+
+// Per-workgroup reduction state for one work item.  Each lane owns 4 columns;
+// each sub-tile brings R rows (wave-uniform) x those 4 columns.
+template <bool BTSG>
+struct Reducer {
+    float best;
+    int bidx;
+    float Mw;  // wave-uniform reference for the exp sums
+    float Stot;
+    float Scol[4];
+
+    __device__ void init() {
+        best = -INFINITY;
+        bidx = 0x7fffffff;
+        Mw = -INFINITY;
+        Stot = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; c++) Scol[c] = 0.0f;
+    }
+
+    // F[r][c]; rowm[r] row index (or -1 when the row is past the map);
+    // coln[c] column index (or -1).  Writes the row partials.
+    template <int R>
+    __device__ void add(const float (&F)[R][4], const int (&rowm)[R], const int (&coln)[4],
+                        uint32_t N_tau, int lane, float2 *rowpart_t, uint32_t n_colblk,
+                        int cb) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (rowm[r] >= 0 && coln[c] >= 0)
+                    argmax_merge(best, bidx, F[r][c], rowm[r] * (int)N_tau + coln[c]);
+        if (!BTSG) return;
+        float lm = -INFINITY;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (rowm[r] >= 0 && coln[c] >= 0) lm = fmaxf(lm, F[r][c]);
+        const float wm = wave_max(lm);
+        if (wm == -INFINITY) return;
+        if (wm > Mw) {
+            const float sc = (Mw == -INFINITY) ? 0.0f : __expf(Mw - wm);
+            Stot *= sc;
+#pragma unroll
+            for (int c = 0; c < 4; c++) Scol[c] *= sc;
+            Mw = wm;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            float sr = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float e = (rowm[r] >= 0 && coln[c] >= 0) ? __expf(F[r][c] - Mw) : 0.0f;
+                sr += e;
+                Scol[c] += e;
+            }
+            Stot += sr;
+            const float srw = wave_sum(sr);
+            if (lane == 0 && rowm[r] >= 0)
+                rowpart_t[(int64_t)rowm[r] * n_colblk + cb] = make_float2(Mw, srw);
+        }
+    }
+};
+
+// End of a work item: combine the 4 waves; write the item partial and the
+// column partials.  colmap(lane, c) gives the local column (0..255) of the
+// lane's c-th column.
+template <bool BTSG, typename ColMap>
+__device__ void finish_item(Reducer<BTSG> &red, int wave, int lane, const TmplDev &td,
+                            const ItemDev &it, int64_t item_id, ItemPart *ipart,
+                            float2 *colpart, ColMap colmap) {
+    __shared__ float sh_best[4];
+    __shared__ int sh_bidx[4];
+    __shared__ float sh_M[4];
+    __shared__ float sh_S[4];
+    __shared__ float sh_col[4][TILE_COLS];
+
+    wave_argmax(red.best, red.bidx);
+    float stot = 0.0f;
+    if (BTSG) stot = wave_sum(red.Stot);
+    if (lane == 0) {
+        sh_best[wave] = red.best;
+        sh_bidx[wave] = red.bidx;
+        sh_M[wave] = red.Mw;
+        sh_S[wave] = stot;
+    }
+    if (BTSG) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) sh_col[wave][colmap(lane, c)] = red.Scol[c];
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    if (BTSG) {
+        const int n = it.cb * TILE_COLS + tid;
+        if (n < (int)td.N_tau) {
+            float M = -INFINITY;
+#pragma unroll
+            for (int w = 0; w < 4; w++) M = fmaxf(M, sh_M[w]);
+            float S = 0.0f;
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (sh_M[w] != -INFINITY) S += sh_col[w][tid] * __expf(sh_M[w] - M);
+            colpart[td.colpart_off + (int64_t)n * td.n_rowblk + it.rb] = make_float2(M, S);
+        }
+    }
+    if (tid == 0) {
+        float bv = sh_best[0];
+        int bi = sh_bidx[0];
+        float M = sh_M[0];
+#pragma unroll
+        for (int w = 1; w < 4; w++) {
+            argmax_merge(bv, bi, sh_best[w], sh_bidx[w]);
+            M = fmaxf(M, sh_M[w]);
+        }
+        float S = 0.0f;
+        if (BTSG) {
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (sh_M[w] != -INFINITY) S += sh_S[w] * __expf(sh_M[w] - M);
+        }
+        ipart[item_id] = ItemPart{bv, bi, M, S};
+    }
+}
+
+// ----------------------------------------------------------------------------
+// rectangular window, regular grid: prefix differences with diagonal reuse
+// ----------------------------------------------------------------------------
+//
+// With dt0 = dtau = a*TAtom and no uint32 wrap-around, the window of cell
+// (m, n) is atoms [i(m), j(m+n)) of the prefix table, where
+//   i(m)   = min(q0 + a*m, N-1)                      (rect.cu:21-31)
+//   j(D)   = min(max(q00 + a*D - 1, 0), N-1) + 1     (rect.cu:54-69), D = m+n,
+// and j >= i always, so the incremental sum of rect.cu:75-91 equals
+// P[j] - P[i].  The item's j(D) range is staged in LDS (component-major,
+// padded every 4 entries so the 4-column lane stride hits distinct banks);
+// each lane computes a 4x4 block whose 16 cells touch 7 diagonals.
+__host__ __device__ __forceinline__ int lds_pad(int d) { return d + (d >> 2); }
+
+template <bool BTSG, bool STRICT>
+__global__ __launch_bounds__(WG) void rect_fast_kernel(
+    const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
+    const double *__restrict__ prefix, float *__restrict__ Fout,
+    ItemPart *__restrict__ ipart, float2 *__restrict__ rowpart,
+    float2 *__restrict__ colpart, int *__restrict__ flags) {
+    extern __shared__ double lds[];
+    const int64_t item_id = item0 + blockIdx.x;
+    const ItemDev it = items[item_id];
+    const TmplDev td = tds[it.t];
+    const int RB = td.rb_rows;
+    const int row0 = it.rb * RB;
+    const int col0 = it.cb * TILE_COLS;
+    const int nd = RB + TILE_COLS - 1;
+    const int ndp = lds_pad(nd) + 1;
+    const int D0 = row0 + col0;
+    const double *__restrict__ P = prefix + td.prefix_off;
+    const int Nm1 = (int)td.N - 1;
+
+    for (int idx = threadIdx.x; idx < nd * NC; idx += WG) {
+        const int d = idx / NC;
+        const int c = idx - d * NC;
+        int64_t q = (int64_t)td.q00 + (int64_t)td.a * (D0 + d) - 1;
+        q = min(max(q, (int64_t)0), (int64_t)Nm1);
+        lds[c * ndp + lds_pad(d)] = P[(q + 1) * NC + c];
+    }
+    __syncthreads();
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    Reducer<BTSG> red;
+    red.init();
+    bool single = false;
+    float *__restrict__ Ft = Fout + td.F_off;
+    float2 *rowpart_t = rowpart + td.rowpart_off;
+
+    int coln[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int n = col0 + lane * 4 + c;
+        coln[c] = n < (int)td.N_tau ? n : -1;
+    }
+
+    for (int s = 0; s < RB / 16; s++) {
+        const int rl0 = s * 16 + wave * 4;
+        if (row0 + rl0 >= (int)td.N_t0) break;
+        const int dbase = rl0 + lane * 4;
+        double w[4][NC];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int k = 0; k < NC; k++) w[c][k] = lds[k * ndp + lds_pad(dbase + c)];
+        float F[4][4];
+        int rowm[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = row0 + rl0 + r;
+            rowm[r] = m < (int)td.N_t0 ? m : -1;
+            if (r > 0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int k = 0; k < NC; k++) w[c][k] = w[c + 1][k];
+#pragma unroll
+                for (int k = 0; k < NC; k++) w[3][k] = lds[k * ndp + lds_pad(dbase + r + 3)];
+            }
+            const int i = min(td.q0 + td.a * m, Nm1);
+            double Pi[NC];
+#pragma unroll
+            for (int k = 0; k < NC; k++) Pi[k] = P[(int64_t)i * NC + k];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                F[r][c] = cell_F(w[c][0] - Pi[0], w[c][1] - Pi[1], w[c][2] - Pi[2],
+                                 w[c][3] - Pi[3], w[c][4] - Pi[4], w[c][5] - Pi[5],
+                                 w[c][6] - Pi[6]);
+                if (STRICT && rowm[r] >= 0 && coln[c] >= 0) {
+                    int q = td.q00 + td.a * (m + coln[c]) - 1;
+                    q = min(max(q, 0), Nm1);
+                    single |= (q == i);
+                }
+            }
+            if (rowm[r] >= 0) {
+                float *Frow = Ft + (int64_t)m * td.N_tau;
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (coln[c] >= 0) Frow[coln[c]] = F[r][c];
+            }
+        }
+        red.template add<4>(F, rowm, coln, td.N_tau, lane, rowpart_t, td.n_colblk, it.cb);
+    }
+    if (STRICT && single) atomicOr(flags + it.t, 1);
+    __syncthreads();
+    finish_item<BTSG>(red, wave, lane, td, it, item_id, ipart, colpart,
+                      [](int ln, int c) { return ln * 4 + c; });
+}
+
+// ----------------------------------------------------------------------------
+// generic per-cell kernels (rect with exact wrap-around semantics; exp)
+// ----------------------------------------------------------------------------
+//
+// Lanes own columns col0 + c*64 + lane (c = 0..3); wave w owns item rows
+// w, w+4, w+8, ...  Row quantities are wave-uniform.
+
+struct RowRect {
+    uint32_t i0;       // window start index
+    uint32_t x0;       // (t0_m + tau - t_min + TAtom/2) mod 2^32 at n = 0
+    uint32_t nwrap;    // first n whose x wrapped past 2^32 (N_tau if none)
+    uint32_t i1_pre;   // i_t1 just before the wrap (running maximum after it)
+};
+
+__device__ __forceinline__ RowRect rect_row(const TmplDev &td, uint32_t m) {
+    RowRect rr;
+    const uint32_t t0m = td.t0 + m * td.dt0;
+    rr.i0 = idx_start(t0m, td.t_min, td.TAtom, td.N);
+    rr.x0 = t0m + td.tau - td.t_min + td.TAtom / 2;
+    const uint64_t room = (1ull << 32) - (uint64_t)rr.x0;  // n*dtau >= room wraps
+    const uint64_t nw = (room + td.dtau - 1) / td.dtau;
+    rr.nwrap = nw < (uint64_t)td.N_tau ? (uint32_t)nw : td.N_tau;
+    rr.i1_pre = 0;
+    if (rr.nwrap < td.N_tau) {
+        const uint32_t xp = rr.x0 + (rr.nwrap - 1) * td.dtau;
+        rr.i1_pre = idx_end_q(xp / td.TAtom, td.N);
+    }
+    return rr;
+}
+
+template <bool BTSG, bool STRICT>
+__global__ __launch_bounds__(WG) void rect_generic_kernel(
+    const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
+    const double *__restrict__ prefix, float *__restrict__ Fout,
+    ItemPart *__restrict__ ipart, float2 *__restrict__ rowpart,
+    float2 *__restrict__ colpart, int *__restrict__ flags) {
+    const int64_t item_id = item0 + blockIdx.x;
+    const ItemDev it = items[item_id];
+    const TmplDev td = tds[it.t];
+    const int RB = td.rb_rows;
+    const int row0 = it.rb * RB;
+    const int col0 = it.cb * TILE_COLS;
+    const double *__restrict__ P = prefix + td.prefix_off;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    Reducer<BTSG> red;
+    red.init();
+    bool single = false;
+    float *__restrict__ Ft = Fout + td.F_off;
+    float2 *rowpart_t = rowpart + td.rowpart_off;
+
+    int coln[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int n = col0 + c * 64 + lane;
+        coln[c] = n < (int)td.N_tau ? n : -1;
+    }
+
+    for (int rl = wave; rl < RB; rl += 4) {
+        const int m = row0 + rl;
+        if (m >= (int)td.N_t0) break;
+        const RowRect rr = rect_row(td, (uint32_t)m);
+        double Pi[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) Pi[k] = P[(int64_t)rr.i0 * NC + k];
+        float F[1][4];
+        int rowm[1] = {m};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t n = coln[c] >= 0 ? (uint32_t)coln[c] : 0u;
+            const uint32_t x = rr.x0 + n * td.dtau;
+            const uint32_t i1 = idx_end_q(x / td.TAtom, td.N);
+            // running maximum of i_t1 over n' <= n (the i_t1_last recursion)
+            const uint32_t run = (n >= rr.nwrap) ? max(i1, rr.i1_pre) : i1;
+            const uint32_t j = max(rr.i0, run + 1);
+            const double *Pj = P + (int64_t)j * NC;
+            F[0][c] = cell_F(Pj[0] - Pi[0], Pj[1] - Pi[1], Pj[2] - Pi[2], Pj[3] - Pi[3],
+                             Pj[4] - Pi[4], Pj[5] - Pi[5], Pj[6] - Pi[6]);
+            if (STRICT && coln[c] >= 0) single |= (i1 == rr.i0);
+            if (coln[c] >= 0) Ft[(int64_t)m * td.N_tau + coln[c]] = F[0][c];
+        }
+        red.template add<1>(F, rowm, coln, td.N_tau, lane, rowpart_t, td.n_colblk, it.cb);
+    }
+    if (STRICT && single) atomicOr(flags + it.t, 1);
+    __syncthreads();
+    finish_item<BTSG>(red, wave, lane, td, it, item_id, ipart, colpart,
+                      [](int ln, int c) { return c * 64 + ln; });
+}
+
+// Exponential window, direct weighted sum over the truncated support
+// (exp.cu:39-102): atoms i in [i_t0, i_t1] with t0 <= t_i <= t1 = t0 + 3 tau,
+// weight w_i = exp(-(t_i - t0)/tau), A/B/C weighted by w^2, Fa/Fb by w.  All
+// lanes of a wave share the row (t0) and therefore the atom stream, which is
+// read once per wave through uniform loads; each lane carries its own tau.
+// The weights are advanced by rho = exp(-TAtom/tau) per atom (FP64, relative
+// error ~ k * 1e-16 after k atoms), the sums are FP64.
+template <bool BTSG, bool STRICT>
+__global__ __launch_bounds__(WG) void exp_direct_kernel(
+    const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
+    const double *__restrict__ merged, float *__restrict__ Fout,
+    ItemPart *__restrict__ ipart, float2 *__restrict__ rowpart,
+    float2 *__restrict__ colpart, int *__restrict__ flags) {
+    const int64_t item_id = item0 + blockIdx.x;
+    const ItemDev it = items[item_id];
+    const TmplDev td = tds[it.t];
+    const int RB = td.rb_rows;
+    const int row0 = it.rb * RB;
+    const int col0 = it.cb * TILE_COLS;
+    const double *__restrict__ X = merged + td.merged_off;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    Reducer<BTSG> red;
+    red.init();
+    bool single = false;
+    float *__restrict__ Ft = Fout + td.F_off;
+    float2 *rowpart_t = rowpart + td.rowpart_off;
+    const uint32_t TA = td.TAtom;
+
+    int coln[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int n = col0 + c * 64 + lane;
+        coln[c] = n < (int)td.N_tau ? n : -1;
+    }
+
+    for (int rl = wave; rl < RB; rl += 4) {
+        const int m = row0 + rl;
+        if (m >= (int)td.N_t0) break;
+        const uint32_t t0m = td.t0 + (uint32_t)m * td.dt0;
+        const uint32_t i0 = idx_start(t0m, td.t_min, TA, td.N);
+        // first atom with t_i >= t0m (timestamps t_min + i*TA never wrap)
+        uint32_t ilo = 0;
+        if (t0m > td.t_min) ilo = (uint32_t)(((uint64_t)(t0m - td.t_min) + TA - 1) / TA);
+        const uint32_t lo = max(i0, ilo);
+        float F[1][4];
+        int rowm[1] = {m};
+        for (int c = 0; c < 4; c++) {
+            const uint32_t n = coln[c] >= 0 ? (uint32_t)coln[c] : 0u;
+            const uint32_t tau = td.tau + n * td.dtau;
+            const uint32_t t1 = t0m + 3u * tau;
+            const uint32_t i1 = idx_end_q((uint32_t)(t1 - td.t_min + TA / 2) / TA, td.N);
+            if (STRICT && coln[c] >= 0) single |= (i1 == i0);
+            // last atom with t_i <= t1
+            int64_t ihi = -1;
+            if (t1 >= td.t_min) ihi = (int64_t)((t1 - td.t_min) / TA);
+            const int64_t hi = min((int64_t)i1, ihi);
+            double acc[NC];
+#pragma unroll
+            for (int k = 0; k < NC; k++) acc[k] = 0.0;
+            const double rtau = 1.0 / (double)tau;
+            const double rho = exp(-(double)TA * rtau);
+            double w = exp(-(double)(td.t_min + lo * TA - t0m) * rtau);
+            // wave-uniform loop bounds: the longest lane decides
+            int64_t hi_w = hi;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) hi_w = max(hi_w, (int64_t)__shfl_xor((long long)hi_w, o));
+            for (int64_t i = lo; i <= hi_w; i++) {
+                const double wi = (i <= hi) ? w : 0.0;
+                const double w2 = wi * wi;
+                const double *xi = X + i * MERGED_STRIDE;
+                acc[0] = fma(xi[0], w2, acc[0]);
+                acc[1] = fma(xi[1], w2, acc[1]);
+                acc[2] = fma(xi[2], w2, acc[2]);
+                acc[3] = fma(xi[3], wi, acc[3]);
+                acc[4] = fma(xi[4], wi, acc[4]);
+                acc[5] = fma(xi[5], wi, acc[5]);
+                acc[6] = fma(xi[6], wi, acc[6]);
+                w *= rho;
+            }
+            F[0][c] = cell_F(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6]);
+            if (coln[c] >= 0) Ft[(int64_t)m * td.N_tau + coln[c]] = F[0][c];
+        }
+        red.template add<1>(F, rowm, coln, td.N_tau, lane, rowpart_t, td.n_colblk, it.cb);
+    }
+    if (STRICT && single) atomicOr(flags + it.t, 1);
+    __syncthreads();
+    finish_item<BTSG>(red, wave, lane, td, it, item_id, ipart, colpart,
+                      [](int ln, int c) { return c * 64 + ln; });
+}
+
+// ----------------------------------------------------------------------------
+// finalize: per-template stats from the partials
+// ----------------------------------------------------------------------------
+
+__device__ void block_argmax(float &bv, int &bi) {
+    __shared__ float sv[WG / 64];
+    __shared__ int si[WG / 64];
+    wave_argmax(bv, bi);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        sv[wave] = bv;
+        si[wave] = bi;
+    }
+    __syncthreads();
+    bv = sv[0];
+    bi = si[0];
+    for (int w = 1; w < WG / 64; w++) argmax_merge(bv, bi, sv[w], si[w]);
+    __syncthreads();
+}
+
+__device__ double block_sum_d(double v) {
+    __shared__ double sd[WG / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sd[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < WG / 64; w++) s += sd[w];
+    __syncthreads();
+    return s;
+}
+
+// argmax over doubles with first-index tie break, block-wide
+__device__ void block_argmax_d(double &bv, int &bi) {
+    __shared__ double sv[WG / 64];
+    __shared__ int si[WG / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double ov = __shfl_xor(bv, o);
+        int oi = __shfl_xor(bi, o);
+        if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        sv[wave] = bv;
+        si[wave] = bi;
+    }
+    __syncthreads();
+    bv = sv[0];
+    bi = si[0];
+    for (int w = 1; w < WG / 64; w++)
+        if (sv[w] > bv || (sv[w] == bv && si[w] < bi)) {
+            bv = sv[w];
+            bi = si[w];
+        }
+    __syncthreads();
+}
+
+template <bool BTSG>
+__global__ __launch_bounds__(WG) void finalize_kernel(
+    const TmplDev *__restrict__ tds, const ItemPart *__restrict__ ipart,
+    const float2 *__restrict__ rowpart, const float2 *__restrict__ colpart,
+    const int *__restrict__ flags, tcw_stats *__restrict__ stats) {
+    const int t = blockIdx.x;
+    const TmplDev td = tds[t];
+    const int64_t nitems = (int64_t)td.n_rowblk * td.n_colblk;
+    float bv = -INFINITY;
+    int bi = 0x7fffffff;
+    for (int64_t k = threadIdx.x; k < nitems; k += WG) {
+        const ItemPart p = ipart[td.item_begin + k];
+        argmax_merge(bv, bi, p.maxF, p.idx);
+    }
+    block_argmax(bv, bi);
+    const float maxF = bv;
+    const int64_t m_ML = bi / (int)td.N_tau;
+    const int64_t n_ML = bi - m_ML * (int64_t)td.N_tau;
+
+    double lnB = NAN, t0_MP = NAN, tau_MP = NAN;
+    int64_t m_MP = -1, n_MP = -1;
+    if (BTSG) {
+        const double ref = (double)maxF;
+        double s = 0.0;
+        for (int64_t k = threadIdx.x; k < nitems; k += WG) {
+            const ItemPart p = ipart[td.item_begin + k];
+            s += (double)p.S * exp((double)p.M - ref);
+        }
+        s = block_sum_d(s);
+        lnB = log(70.0 / ((double)td.N_t0 * (double)td.N_tau)) + ref + log(s);
+        double rv = -1.0;
+        int ri = 0x7fffffff;
+        for (int m = threadIdx.x; m < (int)td.N_t0; m += WG) {
+            double R = 0.0;
+            const float2 *rp = rowpart + td.rowpart_off + (int64_t)m * td.n_colblk;
+            for (uint32_t b = 0; b < td.n_colblk; b++) R += (double)rp[b].y * exp((double)rp[b].x - ref);
+            if (R > rv || (R == rv && m < ri)) {
+                rv = R;
+                ri = m;
+            }
+        }
+        block_argmax_d(rv, ri);
+        m_MP = ri;
+        double cv = -1.0;
+        int ci = 0x7fffffff;
+        for (int n = threadIdx.x; n < (int)td.N_tau; n += WG) {
+            double Cs = 0.0;
+            const float2 *cp = colpart + td.colpart_off + (int64_t)n * td.n_rowblk;
+            for (uint32_t b = 0; b < td.n_rowblk; b++) Cs += (double)cp[b].y * exp((double)cp[b].x - ref);
+            if (Cs > cv || (Cs == cv && n < ci)) {
+                cv = Cs;
+                ci = n;
+            }
+        }
+        block_argmax_d(cv, ci);
+        n_MP = ci;
+        t0_MP = (double)td.o_t0 + ((double)m_MP + 0.5) * ((double)td.o_t0Band / (double)td.N_t0);
+        tau_MP = (double)td.o_tau + ((double)n_MP + 0.5) * ((double)td.o_tauBand / (double)td.N_tau);
+    }
+    if (threadIdx.x == 0) {
+        tcw_stats st;
+        st.maxF = maxF;
+        st.single_atom = flags[t];
+        st.m_ML = m_ML;
+        st.n_ML = n_ML;
+        st.N_t0 = td.N_t0;
+        st.N_tau = td.N_tau;
+        st.t0_ML = (int64_t)td.t0 + m_ML * (int64_t)td.dt0;
+        st.tau_ML = (int64_t)td.tau + n_ML * (int64_t)td.dtau;
+        st.lnBtSG = lnB;
+        st.m_MP = m_MP;
+        st.n_MP = n_MP;
+        st.t0_MP = t0_MP;
+        st.tau_MP = tau_MP;
+        st.numAtoms = td.N;
+        st.t0_data = td.t_min;
+        stats[t] = st;
+    }
+}
+
+__global__ void zero_ints_kernel(int *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define TCW_HIP_TRY(expr)                                                             \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(TCW_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return TCW_OK;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return fail(TCW_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+        }
+        cap = want;
+        return TCW_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+int g_rb_override[MODE_COUNT] = {0, 0, 0};
+
+}  // namespace
+
+struct tcw_plan {
+    int device = 0;
+    uint32_t flags = 0;
+    int64_t T = 0;
+    std::vector<TmplDev> tmpl;
+    std::vector<SegDev> segs;
+    std::vector<ItemDev> items;   // grouped by mode
+    int64_t mode_begin[MODE_COUNT + 1] = {0, 0, 0, 0};
+    int64_t prefix_doubles = 0, merged_doubles = 0, rowpart_n = 0, colpart_n = 0;
+    int64_t F_total = 0;
+    int max_lds_bytes = 0;
+    DevBuf d_tmpl, d_segs, d_items, d_prefix, d_merged, d_ipart, d_rowpart, d_colpart,
+        d_flags;
+    hipGraphExec_t graph = nullptr;
+    hipStream_t cap_stream = nullptr;  // private stream the graph is captured on
+    const void *g_ts = nullptr, *g_cols = nullptr, *g_F = nullptr, *g_stats = nullptr;
+    int64_t g_cstride = -1;
+    void reset() {
+        tmpl.clear();
+        segs.clear();
+        items.clear();
+        prefix_doubles = merged_doubles = rowpart_n = colpart_n = F_total = 0;
+        max_lds_bytes = 0;
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
+    ~tcw_plan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        d_tmpl.release();
+        d_segs.release();
+        d_items.release();
+        d_prefix.release();
+        d_merged.release();
+        d_ipart.release();
+        d_rowpart.release();
+        d_colpart.release();
+        d_flags.release();
+    }
+};
+
+// The handle keeps one plan and one stats slot for single-map calls, reused
+// (grow-only) across calls so tcw_map allocates nothing in steady state.
+struct tcw_handle {
+    int device = 0;
+    tcw_plan single;
+    tcw_stats *d_stats = nullptr;
+    tcw_stats *h_stats = nullptr;  // pinned
+    ~tcw_handle() {
+        if (d_stats) (void)hipFree(d_stats);
+        if (h_stats) (void)hipHostFree(h_stats);
+    }
+};
+
+namespace {
+
+// Effective window and shape (tcw_fstat_map_funcs.py:742-780, on a copy).
+int effective_window(uint32_t numAtoms, uint32_t TAtom, uint32_t t0_data,
+                     const tcw_window &in, tcw_window &w, int64_t &N_t0, int64_t &N_tau) {
+    w = in;
+    if (w.type >= TCW_TRANSIENT_LAST)
+        return fail(TCW_ERR_INVALID, "Unknown window-type (" + std::to_string(w.type) +
+                                         ") passed as input. Allowed are [0," +
+                                         std::to_string(TCW_TRANSIENT_LAST - 1) + "].");
+    if (w.type == TCW_TRANSIENT_NONE) {
+        w.type = TCW_TRANSIENT_RECTANGULAR;
+        w.t0 = t0_data;
+        w.t0Band = 0;
+        w.dt0 = TAtom;
+        w.tau = numAtoms * TAtom;
+        w.tauBand = 0;
+        w.dtau = TAtom;
+    }
+    if (w.dt0 == 0 || w.dtau == 0)
+        return fail(TCW_ERR_INVALID, "windowRange.dt0 and windowRange.dtau must be positive.");
+    N_t0 = (int64_t)(w.t0Band / w.dt0) + 1;
+    N_tau = (int64_t)(w.tauBand / w.dtau) + 1;
+    if (N_t0 * N_tau >= (int64_t)0x7fffffff)
+        return fail(TCW_ERR_INVALID, "map of " + std::to_string(N_t0) + " x " +
+                                         std::to_string(N_tau) + " cells exceeds 2^31-1 cells.");
+    return TCW_OK;
+}
+
+// Build one template's descriptor and items.  seg range and N/t_min precomputed.
+int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
+                  uint32_t TAtom, uint32_t t_min, int64_t seg_begin, int64_t seg_end,
+                  std::vector<ItemDev> (&items_by_mode)[MODE_COUNT]) {
+    tcw_window w;
+    int64_t N_t0, N_tau;
+    int rc = effective_window(N, TAtom, t_min, win_in, w, N_t0, N_tau);
+    if (rc) return rc;
+    TmplDev td;
+    std::memset(&td, 0, sizeof(td));
+    td.seg_begin = seg_begin;
+    td.seg_end = seg_end;
+    td.N = N;
+    td.TAtom = TAtom;
+    td.t_min = t_min;
+    td.type = w.type;
+    td.t0 = w.t0;
+    td.dt0 = w.dt0;
+    td.tau = w.tau;
+    td.dtau = w.dtau;
+    td.N_t0 = (uint32_t)N_t0;
+    td.N_tau = (uint32_t)N_tau;
+    td.o_t0 = win_in.t0;
+    td.o_t0Band = win_in.t0Band;
+    td.o_tau = win_in.tau;
+    td.o_tauBand = win_in.tauBand;
+
+    int mode;
+    if (w.type == TCW_TRANSIENT_EXPONENTIAL) {
+        mode = MODE_EXP_DIRECT;
+    } else {
+        mode = MODE_RECT_GENERIC;
+        const uint64_t half = TAtom / 2;
+        const bool regular = (w.dt0 % TAtom == 0) && (w.dtau % TAtom == 0) && (w.dt0 == w.dtau);
+        const uint64_t x_lo = (uint64_t)w.t0 + half;  // must not be below t_min
+        const uint64_t x_hi = (uint64_t)w.t0 + w.t0Band + w.tau + w.tauBand + half;
+        if (!(p.flags & TCW_FLAG_GENERIC) && regular && x_lo >= t_min && x_hi < (1ull << 32)) {
+            mode = MODE_RECT_FAST;
+            td.a = (int32_t)(w.dt0 / TAtom);
+            td.q0 = (int32_t)(((uint64_t)w.t0 - t_min + half) / TAtom);
+            td.q00 = (int32_t)(((uint64_t)w.t0 + w.tau - t_min + half) / TAtom);
+        }
+    }
+    td.mode = mode;
+    int rb = g_rb_override[mode] > 0 ? g_rb_override[mode] : RB_DEFAULT[mode];
+    td.rb_rows = rb;
+    td.n_rowblk = (uint32_t)((N_t0 + rb - 1) / rb);
+    td.n_colblk = (uint32_t)((N_tau + TILE_COLS - 1) / TILE_COLS);
+    if (mode == MODE_EXP_DIRECT) {
+        td.merged_off = p.merged_doubles;
+        p.merged_doubles += (int64_t)N * MERGED_STRIDE;
+    } else {
+        td.prefix_off = p.prefix_doubles;
+        p.prefix_doubles += ((int64_t)N + 1) * NC;
+    }
+    td.F_off = p.F_total;
+    p.F_total += N_t0 * N_tau;
+    td.rowpart_off = p.rowpart_n;
+    p.rowpart_n += N_t0 * (int64_t)td.n_colblk;
+    td.colpart_off = p.colpart_n;
+    p.colpart_n += N_tau * (int64_t)td.n_rowblk;
+    // items of a template are contiguous in item-id space; item_begin fixed below
+    for (uint32_t rbk = 0; rbk < td.n_rowblk; rbk++)
+        for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
+            items_by_mode[mode].push_back(ItemDev{(int32_t)t, (int32_t)rbk, (int32_t)cbk, 0});
+    if (mode == MODE_RECT_FAST) {
+        const int nd = rb + TILE_COLS - 1;
+        const int bytes = (int)sizeof(double) * NC * (lds_pad(nd) + 1);
+        p.max_lds_bytes = std::max(p.max_lds_bytes, bytes);
+    }
+    p.tmpl.push_back(td);
+    return TCW_OK;
+}
+
+// After all templates: lay out items mode by mode, fix item_begin (the items
+// of one template stay contiguous, rb-major), size scratch, upload descriptors
+// (asynchronously on `st` when given, else synchronously).
+int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
+                hipStream_t st, bool async) {
+    p.items.clear();
+    for (int md = 0; md < MODE_COUNT; md++) {
+        p.mode_begin[md] = (int64_t)p.items.size();
+        for (size_t k = 0; k < items_by_mode[md].size(); k++) {
+            const ItemDev &it = items_by_mode[md][k];
+            TmplDev &td = p.tmpl[it.t];
+            if (it.rb == 0 && it.cb == 0) td.item_begin = (int64_t)p.items.size();
+            p.items.push_back(it);
+        }
+    }
+    p.mode_begin[MODE_COUNT] = (int64_t)p.items.size();
+    int rc;
+    if ((rc = p.d_tmpl.ensure(sizeof(TmplDev) * p.tmpl.size()))) return rc;
+    if ((rc = p.d_segs.ensure(sizeof(SegDev) * std::max<size_t>(p.segs.size(), 1)))) return rc;
+    if ((rc = p.d_items.ensure(sizeof(ItemDev) * std::max<size_t>(p.items.size(), 1)))) return rc;
+    if ((rc = p.d_prefix.ensure(sizeof(double) * std::max<int64_t>(p.prefix_doubles, 1)))) return rc;
+    if ((rc = p.d_merged.ensure(sizeof(double) * std::max<int64_t>(p.merged_doubles, 1)))) return rc;
+    if ((rc = p.d_ipart.ensure(sizeof(ItemPart) * std::max<size_t>(p.items.size(), 1)))) return rc;
+    if ((rc = p.d_rowpart.ensure(sizeof(float2) * std::max<int64_t>(p.rowpart_n, 1)))) return rc;
+    if ((rc = p.d_colpart.ensure(sizeof(float2) * std::max<int64_t>(p.colpart_n, 1)))) return rc;
+    if ((rc = p.d_flags.ensure(sizeof(int) * std::max<int64_t>(p.T, 1)))) return rc;
+    auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+        if (bytes == 0) return hipSuccess;
+        return async ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
+                     : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    TCW_HIP_TRY(up(p.d_tmpl.p, p.tmpl.data(), sizeof(TmplDev) * p.tmpl.size()));
+    TCW_HIP_TRY(up(p.d_segs.p, p.segs.data(), sizeof(SegDev) * p.segs.size()));
+    TCW_HIP_TRY(up(p.d_items.p, p.items.data(), sizeof(ItemDev) * p.items.size()));
+    return TCW_OK;
+}
+
+// Plan `d` into `p` (buffers reused, grow-only).
+int plan_build(tcw_plan &p, int device, const tcw_batch_desc *d, uint32_t flags,
+               int64_t *F_off, hipStream_t st, bool async) {
+    if (d->T <= 0) return fail(TCW_ERR_INVALID, "no templates");
+    if (d->TAtom < 2) return fail(TCW_ERR_INVALID, "TAtom must be at least 2 seconds.");
+    p.reset();
+    p.device = device;
+    p.flags = flags;
+    p.T = d->T;
+    std::vector<ItemDev> items_by_mode[MODE_COUNT];
+    for (int64_t t = 0; t < d->T; t++) {
+        const int64_t s0 = d->seg_begin[t], s1 = d->seg_begin[t + 1];
+        if (s1 <= s0 || s1 - s0 > TCW_MAX_DETECTORS)
+            return fail(TCW_ERR_INVALID, "template " + std::to_string(t) + ": needs 1.." +
+                                             std::to_string(TCW_MAX_DETECTORS) +
+                                             " detector segments");
+        uint32_t tmin = 0xffffffffu, tmax = 0;
+        const int64_t first_seg = (int64_t)p.segs.size();
+        for (int64_t s = s0; s < s1; s++) {
+            if (d->seg_off[s + 1] <= d->seg_off[s])
+                return fail(TCW_ERR_INVALID, "template " + std::to_string(t) + ": empty detector");
+            if (d->seg_tlast[s] < d->seg_tfirst[s])
+                return fail(TCW_ERR_INVALID, "template " + std::to_string(t) +
+                                                 ": timestamps not in ascending order");
+            tmin = std::min(tmin, d->seg_tfirst[s]);
+            tmax = std::max(tmax, d->seg_tlast[s]);
+            p.segs.push_back(SegDev{d->seg_off[s], d->seg_off[s + 1]});
+        }
+        const uint64_t N = (uint64_t)(tmax - tmin) / d->TAtom + 1;
+        if (F_off) F_off[t] = p.F_total;
+        int rc = plan_template(p, t, d->windows[t], (uint32_t)N, d->TAtom, tmin, first_seg,
+                               (int64_t)p.segs.size(), items_by_mode);
+        if (rc) return rc;
+    }
+    if (F_off) F_off[d->T] = p.F_total;
+    return plan_finish(p, items_by_mode, st, async);
+}
+
+template <bool BTSG, bool STRICT>
+void launch_maps(tcw_plan &p, float *F, hipStream_t st) {
+    auto *tds = (const TmplDev *)p.d_tmpl.p;
+    auto *items = (const ItemDev *)p.d_items.p;
+    auto *prefix = (const double *)p.d_prefix.p;
+    auto *merged = (const double *)p.d_merged.p;
+    auto *ipart = (ItemPart *)p.d_ipart.p;
+    auto *rowpart = (float2 *)p.d_rowpart.p;
+    auto *colpart = (float2 *)p.d_colpart.p;
+    auto *flags = (int *)p.d_flags.p;
+    int64_t n;
+    if ((n = p.mode_begin[MODE_RECT_FAST + 1] - p.mode_begin[MODE_RECT_FAST]) > 0)
+        hipLaunchKernelGGL((rect_fast_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG),
+                           p.max_lds_bytes, st, tds, items, p.mode_begin[MODE_RECT_FAST],
+                           prefix, F, ipart, rowpart, colpart, flags);
+    if ((n = p.mode_begin[MODE_RECT_GENERIC + 1] - p.mode_begin[MODE_RECT_GENERIC]) > 0)
+        hipLaunchKernelGGL((rect_generic_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG), 0,
+                           st, tds, items, p.mode_begin[MODE_RECT_GENERIC], prefix, F, ipart,
+                           rowpart, colpart, flags);
+    if ((n = p.mode_begin[MODE_EXP_DIRECT + 1] - p.mode_begin[MODE_EXP_DIRECT]) > 0)
+        hipLaunchKernelGGL((exp_direct_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG), 0,
+                           st, tds, items, p.mode_begin[MODE_EXP_DIRECT], merged, F, ipart,
+                           rowpart, colpart, flags);
+}
+
+// Optional per-stage event marks: ev[0] before the stage kernel, ev[1] after
+// it, ev[2] after the map kernels, ev[3] after finalize.
+// check_errors = false while a stream is being captured (hipGetLastError is
+// not permitted then; launch errors surface at instantiate/launch instead).
+int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstride,
+               float *F, tcw_stats *stats_dev, hipStream_t st, hipEvent_t *ev = nullptr,
+               bool check_errors = true) {
+    const bool btsg = p.flags & TCW_FLAG_BTSG;
+    const bool strict = p.flags & TCW_FLAG_LAL_STRICT;
+    auto *flags = (int *)p.d_flags.p;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)((p.T + 255) / 256)), dim3(256), 0, st,
+                       flags, p.T);
+    if (ev) TCW_HIP_TRY(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(WG), 0, st,
+                       (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
+                       cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
+    if (ev) TCW_HIP_TRY(hipEventRecord(ev[1], st));
+    if (btsg) {
+        if (strict)
+            launch_maps<true, true>(p, F, st);
+        else
+            launch_maps<true, false>(p, F, st);
+        if (ev) TCW_HIP_TRY(hipEventRecord(ev[2], st));
+        hipLaunchKernelGGL(finalize_kernel<true>, dim3((unsigned)p.T), dim3(WG), 0, st,
+                           (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
+                           (const float2 *)p.d_rowpart.p, (const float2 *)p.d_colpart.p,
+                           (const int *)p.d_flags.p, stats_dev);
+    } else {
+        if (strict)
+            launch_maps<false, true>(p, F, st);
+        else
+            launch_maps<false, false>(p, F, st);
+        if (ev) TCW_HIP_TRY(hipEventRecord(ev[2], st));
+        hipLaunchKernelGGL(finalize_kernel<false>, dim3((unsigned)p.T), dim3(WG), 0, st,
+                           (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
+                           (const float2 *)p.d_rowpart.p, (const float2 *)p.d_colpart.p,
+                           (const int *)p.d_flags.p, stats_dev);
+    }
+    if (ev) TCW_HIP_TRY(hipEventRecord(ev[3], st));
+    if (check_errors) TCW_HIP_TRY(hipGetLastError());
+    return TCW_OK;
+}
+
+int set_device(int device) {
+    int cur = -1;
+    TCW_HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) TCW_HIP_TRY(hipSetDevice(device));
+    return TCW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tcw_version(void) { return "tcw_hip 0.1 (gfx950)"; }
+
+const char *tcw_last_error(void) { return g_err.c_str(); }
+
+int tcw_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int tcw_device_name(int device, char *buf, int buflen) {
+    if (!buf || buflen <= 0) return fail(TCW_ERR_INVALID, "tcw_device_name: no buffer");
+    hipDeviceProp_t prop;
+    TCW_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    std::snprintf(buf, (size_t)buflen, "%s", prop.name);
+    return TCW_OK;
+}
+
+int tcw_map_shape(uint32_t numAtoms, uint32_t TAtom, uint32_t t0_data, const tcw_window *win,
+                  int64_t *N_t0, int64_t *N_tau) {
+    if (!win || !N_t0 || !N_tau) return fail(TCW_ERR_INVALID, "tcw_map_shape: null argument");
+    tcw_window w;
+    return effective_window(numAtoms, TAtom, t0_data, *win, w, *N_t0, *N_tau);
+}
+
+int tcw_create(int device, tcw_handle **out) {
+    if (!out) return fail(TCW_ERR_INVALID, "tcw_create: null output");
+    int n = tcw_device_count();
+    if (device < 0 || device >= n)
+        return fail(TCW_ERR_INVALID, "Requested HIP device number " + std::to_string(device) +
+                                         " exceeds number of available devices (" +
+                                         std::to_string(n) + ").");
+    int rc = set_device(device);
+    if (rc) return rc;
+    auto *h = new tcw_handle;
+    h->device = device;
+    if (hipMalloc(&h->d_stats, sizeof(tcw_stats)) != hipSuccess ||
+        hipHostMalloc(&h->h_stats, sizeof(tcw_stats), hipHostMallocDefault) != hipSuccess) {
+        delete h;
+        return fail(TCW_ERR_NOMEM, "tcw_create: stats slot allocation failed");
+    }
+    *out = h;
+    return TCW_OK;
+}
+
+int tcw_destroy(tcw_handle *h) {
+    if (h) (void)set_device(h->device);
+    delete h;
+    return TCW_OK;
+}
+
+int tcw_batch_plan(tcw_handle *h, const tcw_batch_desc *d, uint32_t flags, tcw_plan **out,
+                   int64_t *F_off) {
+    if (!h || !d || !out) return fail(TCW_ERR_INVALID, "tcw_batch_plan: null argument");
+    int rc = set_device(h->device);
+    if (rc) return rc;
+    auto *p = new tcw_plan;
+    rc = plan_build(*p, h->device, d, flags, F_off, nullptr, false);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return TCW_OK;
+}
+
+int64_t tcw_plan_scratch_bytes(const tcw_plan *p) {
+    if (!p) return 0;
+    return (int64_t)(p->d_tmpl.cap + p->d_segs.cap + p->d_items.cap + p->d_prefix.cap +
+                     p->d_merged.cap + p->d_ipart.cap + p->d_rowpart.cap + p->d_colpart.cap +
+                     p->d_flags.cap);
+}
+
+int tcw_batch_run(tcw_handle *h, tcw_plan *p, const uint32_t *ts, const float *cols,
+                  int64_t col_stride, float *F_out, tcw_stats *stats_dev, int use_graph,
+                  void *stream) {
+    if (!h || !p || !ts || !cols || !F_out || !stats_dev)
+        return fail(TCW_ERR_INVALID, "tcw_batch_run: null argument");
+    int rc = set_device(p->device);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if (!use_graph) return launch_all(*p, ts, cols, col_stride, F_out, stats_dev, st);
+    const bool same = p->graph && p->g_ts == ts && p->g_cols == cols && p->g_F == F_out &&
+                      p->g_stats == stats_dev && p->g_cstride == col_stride;
+    if (!same) {
+        if (p->graph) {
+            (void)hipGraphExecDestroy(p->graph);
+            p->graph = nullptr;
+        }
+        // Capture on a private stream: the caller's stream may be the legacy
+        // default stream, which cannot be captured.  The graph is then
+        // launched on the caller's stream.
+        if (!p->cap_stream)
+            TCW_HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        TCW_HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        rc = launch_all(*p, ts, cols, col_stride, F_out, stats_dev, p->cap_stream, nullptr,
+                        false);
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &g);  // always end the capture
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return fail(TCW_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            p->graph = nullptr;
+            return fail(TCW_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        p->g_ts = ts;
+        p->g_cols = cols;
+        p->g_F = F_out;
+        p->g_stats = stats_dev;
+        p->g_cstride = col_stride;
+    }
+    TCW_HIP_TRY(hipGraphLaunch(p->graph, st));
+    return TCW_OK;
+}
+
+int tcw_batch_run_timed(tcw_handle *h, tcw_plan *p, const uint32_t *ts, const float *cols,
+                        int64_t col_stride, float *F_out, tcw_stats *stats_dev,
+                        double *ms_out, void *stream) {
+    if (!h || !p || !ts || !cols || !F_out || !stats_dev || !ms_out)
+        return fail(TCW_ERR_INVALID, "tcw_batch_run_timed: null argument");
+    int rc = set_device(p->device);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; k++) TCW_HIP_TRY(hipEventCreate(&ev[k]));
+    rc = launch_all(*p, ts, cols, col_stride, F_out, stats_dev, st, ev);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipEventSynchronize(ev[3]);
+    if (!rc && e == hipSuccess) {
+        float a = 0, b = 0, c = 0;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+        ms_out[0] = a;
+        ms_out[1] = b;
+        ms_out[2] = c;
+    }
+    for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TCW_ERR_DEVICE, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+    return TCW_OK;
+}
+
+int tcw_plan_destroy(tcw_plan *p) {
+    delete p;
+    return TCW_OK;
+}
+
+int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win, float *F_mn_dev,
+            tcw_stats *stats, uint32_t flags, void *stream) {
+    if (!h || !atoms || !win || !F_mn_dev || !stats)
+        return fail(TCW_ERR_INVALID, "tcw_map: null argument");
+    if (atoms->numDet < 1 || atoms->numDet > TCW_MAX_DETECTORS)
+        return fail(TCW_ERR_INVALID,
+                    "tcw_map: numDet must be 1.." + std::to_string(TCW_MAX_DETECTORS));
+    if (!atoms->ts || !atoms->cols) return fail(TCW_ERR_INVALID, "tcw_map: null atoms");
+    if (atoms->t_max < atoms->t_min)
+        return fail(TCW_ERR_INVALID, "tcw_map: t_max < t_min");
+    int rc = set_device(h->device);
+    if (rc) return rc;
+    int64_t seg_begin[2] = {0, (int64_t)atoms->numDet};
+    uint32_t tfirst[TCW_MAX_DETECTORS], tlast[TCW_MAX_DETECTORS];
+    // only the overall min/max matter for the binned length
+    for (uint32_t X = 0; X < atoms->numDet; X++) {
+        tfirst[X] = atoms->t_min;
+        tlast[X] = atoms->t_max;
+    }
+    tcw_batch_desc d;
+    d.T = 1;
+    d.TAtom = atoms->TAtom;
+    d.seg_begin = seg_begin;
+    d.seg_off = atoms->det_off;
+    d.seg_tfirst = tfirst;
+    d.seg_tlast = tlast;
+    d.windows = win;
+    hipStream_t st = (hipStream_t)stream;
+    rc = plan_build(h->single, h->device, &d, flags, nullptr, st, true);
+    if (rc) return rc;
+    rc = launch_all(h->single, atoms->ts, atoms->cols, atoms->col_stride, F_mn_dev,
+                    h->d_stats, st);
+    if (rc) return rc;
+    TCW_HIP_TRY(hipMemcpyAsync(h->h_stats, h->d_stats, sizeof(tcw_stats),
+                               hipMemcpyDeviceToHost, st));
+    TCW_HIP_TRY(hipStreamSynchronize(st));
+    *stats = *h->h_stats;
+    if ((flags & TCW_FLAG_LAL_STRICT) && stats->single_atom)
+        return fail(TCW_ERR_SINGLE_ATOM,
+                    "encountered a single-atom Fstat-calculation. This is degenerate and cannot "
+                    "be computed! t0 must stay away *at least* 2*TAtom from the end of the data.");
+    return TCW_OK;
+}
+
+// Testing/tuning hook: rows per work item for a kernel mode (0 restores the default).
+int tcw_set_rows_per_item(int mode, int rows) {
+    if (mode < 0 || mode >= MODE_COUNT) return fail(TCW_ERR_INVALID, "bad mode");
+    if (rows != 0 && (rows < 16 || rows > 512 || rows % 16))
+        return fail(TCW_ERR_INVALID, "rows per item must be a multiple of 16 in [16, 512]");
+    g_rb_override[mode] = rows;
+    return TCW_OK;
+}
+
+}  // extern "C"
