@@ -93,52 +93,54 @@ def plain_loop(b, batch, idx, BtSG):
 
 
 def cpu_baseline(batch, budget_s, seed=5):
-    """FP64 C oracle, OpenMP over templates, on a seeded sample of templates."""
+    """FP64 C oracle, OpenMP over templates, on a seeded sample of templates
+    sized to about budget_s seconds of all-core CPU time."""
     import oracle
 
     cores = len(os.sched_getaffinity(0))
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp:
         cores = min(cores, int(omp))
-    rng = np.random.default_rng(seed)
-    order = rng.permutation(batch.T)
-    merged, TAs, t0s, wins, cells = [], [], [], [], []
-    for t in order:
-        a, e = int(batch.off[t]), int(batch.off[t + 1])
-        m, t0d = oracle.merge_binned(batch.ts[a:e], batch.cols[:, a:e],
-                                     np.array([0, e - a]), batch.TAtom)
-        merged.append(m)
-        TAs.append(batch.TAtom)
-        t0s.append(t0d)
-        wins.append(batch.windows[t])
-        w = batch.windows[t]
-        cells.append((w.t0Band // w.dt0 + 1) * (w.tauBand // w.dtau + 1))
-        if len(merged) >= 4 * cores and len(merged) % cores == 0:
-            break
-    # calibrate on `cores` templates, then size the sample to the budget
-    n0 = cores
-    t0 = time.perf_counter()
-    oracle.compute_map_batch(merged[:n0], TAs[:n0], t0s[:n0], wins[:n0], with_lnBtSG=True)
-    dt = time.perf_counter() - t0
-    per_cell = dt / max(sum(cells[:n0]), 1)
-    want_cells = budget_s / per_cell
-    k = n0
-    while k < len(merged) and sum(cells[:k]) < want_cells:
-        k += 1
-    if k > n0:
+    order = np.random.default_rng(seed).permutation(batch.T)
+
+    def prepare(ts_idx):
+        out = []
+        for t in ts_idx:
+            a, e = int(batch.off[t]), int(batch.off[t + 1])
+            m, t0d = oracle.merge_binned(batch.ts[a:e], batch.cols[:, a:e],
+                                         np.array([0, e - a]), batch.TAtom)
+            w = batch.windows[t]
+            out.append((m, t0d, w, (w.t0Band // w.dt0 + 1) * (w.tauBand // w.dtau + 1)))
+        return out
+
+    def run(sample):
         t0 = time.perf_counter()
-        oracle.compute_map_batch(merged[:k], TAs[:k], t0s[:k], wins[:k], with_lnBtSG=True)
-        dt = time.perf_counter() - t0
-    else:
-        k = n0
-    ncells = sum(cells[:k])
+        oracle.compute_map_batch([x[0] for x in sample], [batch.TAtom] * len(sample),
+                                 [x[1] for x in sample], [x[2] for x in sample],
+                                 with_lnBtSG=True)
+        return time.perf_counter() - t0
+
+    sample = prepare(order[:cores])  # calibration: one template per core
+    dt = run(sample)
+    rate = sum(x[3] for x in sample) / dt
+    want = rate * budget_s
+    k = cores
+    total = sum(x[3] for x in sample)
+    all_cells = batch.cells()
+    while k < batch.T and total < want:
+        k += 1
+        total += int(all_cells[order[k - 1]])
+    if k > cores:
+        sample = prepare(order[:k])
+        dt = run(sample)
+    ncells = sum(x[3] for x in sample)
     return {
         "value": ncells / dt,
         "unit": "cells/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{k} seeded-random templates of the workload ({ncells:.3e} cells, "
-                  f"{dt:.1f} s), FP64 C oracle with lnBtSG, OpenMP over templates",
+        "sample": f"{len(sample)} seeded-random templates of the workload ({ncells:.3e} "
+                  f"cells, {dt:.1f} s), FP64 C oracle with lnBtSG, OpenMP over templates",
     }
 
 
@@ -186,6 +188,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-templates", type=int, default=6)
+    ap.add_argument("--profile-mode", action="store_true",
+                    help="for rocprofv3: a few ungraphed steps plus a 1 GiB device copy "
+                         "(reference for counter scaling); no baselines")
     ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
                     metavar=("FETCH_KIB", "WRITE_KIB"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE of the map kernel (KiB per "
@@ -215,6 +220,19 @@ def main():
     cells = b.total_cells
     atoms_binned = int(batch.n_sft.sum())
     log(f"plan: {cells} cells, scratch {b.scratch_bytes() / 2**30:.2f} GiB")
+
+    if args.profile_mode:
+        for _ in range(args.warmup + args.steps):
+            b.run(use_graph=False)
+        torch.cuda.synchronize()
+        ref = torch.empty(2**28, dtype=torch.float32, device=b.device)
+        ref2 = torch.empty_like(ref)
+        for _ in range(3):
+            ref2.copy_(ref)  # 1 GiB read + 1 GiB write, the counters' yardstick
+        torch.cuda.synchronize()
+        log("profile mode done: map steps + 3 x 1 GiB copies")
+        b.close()
+        return
 
     for _ in range(args.warmup):
         b.run(use_graph=use_graph)

@@ -52,7 +52,7 @@ enum Mode : int32_t {
     MODE_COUNT = 3
 };
 
-constexpr int RB_DEFAULT[MODE_COUNT] = {64, 16, 16};
+constexpr int RB_DEFAULT[MODE_COUNT] = {128, 16, 16};
 
 struct TmplDev {
     int64_t seg_begin, seg_end;  // detector segments of this template
@@ -60,8 +60,8 @@ struct TmplDev {
     int64_t prefix_off;          // doubles, prefix entries [N+1][7]  (rect modes)
     int64_t F_off;               // floats, map [N_t0][N_tau]
     int64_t item_begin;          // first work item of this template
-    int64_t rowpart_off;         // float2, [N_t0][n_colblk]
-    int64_t colpart_off;         // float2, [N_tau][n_rowblk]
+    int64_t rowpart_off;         // float, [N_t0][n_colblk], relative to the item max
+    int64_t colpart_off;         // float, [N_tau][n_rowblk], relative to the item max
     uint32_t N, TAtom, t_min;    // binned atoms, atom length, t0_data
     uint32_t type, t0, dt0, tau, dtau;  // effective window (after NONE -> RECT)
     uint32_t N_t0, N_tau;
@@ -89,7 +89,7 @@ struct ItemDev {
 struct ItemPart {
     float maxF;
     int32_t idx;
-    float M, S;  // sum of exp(F - M) over the item's cells
+    float M, S;  // M = the item's max F; S = sum of exp(F - M) over its cells
 };
 
 // ----------------------------------------------------------------------------
@@ -122,12 +122,6 @@ __device__ __forceinline__ df df_add(df a, df b) {
     return fast_two_sum(s.hi, s.lo);
 }
 
-__device__ __forceinline__ df df_add_f(df a, float b) {
-    df s = two_sum(a.hi, b);
-    s.lo += a.lo;
-    return fast_two_sum(s.hi, s.lo);
-}
-
 // exact value of a (hi, lo) pair: 2 x 24 bits fit a double
 __device__ __forceinline__ double df_value(df a) { return (double)a.hi + (double)a.lo; }
 
@@ -135,12 +129,28 @@ __device__ __forceinline__ double df_value(df a) { return (double)a.hi + (double
 // stage: binned merge + compensated prefix (rect) / FP64 atom table (exp)
 // ----------------------------------------------------------------------------
 
-// One workgroup per template.  Thread k owns a contiguous run of bins.  Each
-// bin's merged value is the float32 sum over detectors X = 0,1,... (in that
-// order) of the detector's atoms in the bin (in atom order), starting from 0 --
-// the same additions, in the same order, as lalpulsar's binned merge, so the
-// merged atoms are bit-identical.
-__global__ __launch_bounds__(WG) void stage_kernel(
+constexpr int STAGE_WG = 1024;   // 16 wave64
+constexpr int STAGE_BINS = 8;    // consecutive bins per thread per round
+constexpr int STAGE_ROUND = STAGE_WG * STAGE_BINS;
+
+__device__ __forceinline__ df shfl_up_df(df v, int d) {
+    return {__shfl_up(v.hi, d), __shfl_up(v.lo, d)};
+}
+
+__device__ __forceinline__ df df_from(float x) { return {x, 0.0f}; }
+
+// One workgroup per template, rounds of 8192 bins, 8 consecutive bins per
+// thread.  A bin's merged value is the float32 sum over detectors X = 0,1,...
+// (in that order) of the detector's atoms in the bin (in atom order), starting
+// from 0 -- the same additions, in the same order, as lalpulsar's binned
+// merge, so the merged atoms are bit-identical.  Rect templates then get the
+// exclusive prefix P[b] = sum of bins < b per column as a double-float (FP32
+// two-sum) scan: running sums over the thread's 8 bins, a Kogge-Stone scan of
+// the thread totals within the wave, one thread per column scanning the 16
+// wave totals (plus the carry of earlier rounds) in LDS.  Each P[b] is stored
+// as the exact FP64 value hi + lo, [N+1][7].  Exp templates get the merged
+// atoms as an FP64 table [N][8].
+__global__ __launch_bounds__(STAGE_WG) void stage_kernel(
     const TmplDev *__restrict__ tds, const SegDev *__restrict__ segs,
     const uint32_t *__restrict__ ts, const float *__restrict__ cols, int64_t cstride,
     double *__restrict__ prefix, double *__restrict__ merged) {
@@ -148,114 +158,113 @@ __global__ __launch_bounds__(WG) void stage_kernel(
     const bool want_prefix = td.mode != MODE_EXP_DIRECT;
     const uint32_t N = td.N, TA = td.TAtom;
     const int tid = threadIdx.x;
-    const uint32_t chunk = (N + WG - 1) / WG;
-    const uint32_t b0 = min(N, (uint32_t)tid * chunk);
-    const uint32_t b1 = min(N, b0 + chunk);
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int nseg = (int)(td.seg_end - td.seg_begin);
-
-    int64_t cur0[TCW_MAX_DETECTORS], cur[TCW_MAX_DETECTORS], endk[TCW_MAX_DETECTORS];
-    const uint64_t tstart = (uint64_t)td.t_min + (uint64_t)b0 * TA;
-#pragma unroll
-    for (int X = 0; X < TCW_MAX_DETECTORS; X++) {
-        cur0[X] = 0;
-        endk[X] = 0;
-        if (X < nseg) {
-            const SegDev sg = segs[td.seg_begin + X];
-            int64_t lo = sg.begin, hi = sg.end;  // first atom with ts >= tstart
-            while (lo < hi) {
-                int64_t mid = (lo + hi) >> 1;
-                if ((uint64_t)ts[mid] < tstart)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            cur0[X] = lo;
-            endk[X] = sg.end;
-        }
+    __shared__ float sh_hi[NC][STAGE_WG / 64 + 1];  // wave totals, then offsets
+    __shared__ float sh_lo[NC][STAGE_WG / 64 + 1];  // [.][16] = running carry
+    if (tid < NC) {
+        sh_hi[tid][STAGE_WG / 64] = 0.0f;
+        sh_lo[tid][STAGE_WG / 64] = 0.0f;
     }
+    double *P = prefix + td.prefix_off;
+    double *Xt = merged + td.merged_off;
 
-    auto merge_bin = [&](uint32_t b, float v[NC]) {
+    for (uint32_t base = 0; base < N; base += STAGE_ROUND) {
+        const uint32_t b0 = base + (uint32_t)tid * STAGE_BINS;
+        float v[STAGE_BINS][NC];
 #pragma unroll
-        for (int c = 0; c < NC; c++) v[c] = 0.0f;
-        const uint64_t lim = (uint64_t)td.t_min + (uint64_t)(b + 1) * TA;
+        for (int j = 0; j < STAGE_BINS; j++)
 #pragma unroll
-        for (int X = 0; X < TCW_MAX_DETECTORS; X++) {
-            if (X < nseg) {
-                while (cur[X] < endk[X] && (uint64_t)ts[cur[X]] < lim) {
+            for (int c = 0; c < NC; c++) v[j][c] = 0.0f;
+        if (b0 < N) {
+            const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b0 * TA;
+            for (int X = 0; X < nseg; X++) {
+                const SegDev sg = segs[td.seg_begin + X];
+                int64_t lo = sg.begin, hi = sg.end;  // first atom with ts >= t_lo
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if ((uint64_t)ts[mid] < t_lo)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                int64_t k = lo;
 #pragma unroll
-                    for (int c = 0; c < NC; c++) v[c] += cols[c * cstride + cur[X]];
-                    cur[X]++;
+                for (int j = 0; j < STAGE_BINS; j++) {
+                    const uint64_t t_hi = t_lo + (uint64_t)(j + 1) * TA;
+                    for (; k < sg.end && (uint64_t)ts[k] < t_hi; k++) {
+#pragma unroll
+                        for (int c = 0; c < NC; c++) v[j][c] += cols[c * cstride + k];
+                    }
                 }
             }
         }
-    };
-
+        if (!want_prefix) {
 #pragma unroll
-    for (int X = 0; X < TCW_MAX_DETECTORS; X++) cur[X] = cur0[X];
-
-    if (!want_prefix) {
-        double *out = merged + td.merged_off;
-        for (uint32_t b = b0; b < b1; b++) {
-            float v[NC];
-            merge_bin(b, v);
+            for (int j = 0; j < STAGE_BINS; j++) {
+                const uint32_t b = b0 + j;
+                if (b < N) {
 #pragma unroll
-            for (int c = 0; c < NC; c++) out[(int64_t)b * MERGED_STRIDE + c] = (double)v[c];
-            out[(int64_t)b * MERGED_STRIDE + NC] = 0.0;
+                    for (int c = 0; c < NC; c++) Xt[(int64_t)b * MERGED_STRIDE + c] = (double)v[j][c];
+                    Xt[(int64_t)b * MERGED_STRIDE + NC] = 0.0;
+                }
+            }
+            continue;
         }
-        return;
-    }
-
-    // pass 1: compensated totals of this thread's bins
-    df tot[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) tot[c] = {0.0f, 0.0f};
-    for (uint32_t b = b0; b < b1; b++) {
-        float v[NC];
-        merge_bin(b, v);
-#pragma unroll
-        for (int c = 0; c < NC; c++) tot[c] = df_add_f(tot[c], v[c]);
-    }
-
-    // exclusive scan of the 256 thread totals, one column per thread
-    __shared__ float sh_hi[NC][WG];
-    __shared__ float sh_lo[NC][WG];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        sh_hi[c][tid] = tot[c].hi;
-        sh_lo[c][tid] = tot[c].lo;
-    }
-    __syncthreads();
-    if (tid < NC) {
-        df run = {0.0f, 0.0f};
-        for (int k = 0; k < WG; k++) {
-            df x = {sh_hi[tid][k], sh_lo[tid][k]};
-            sh_hi[tid][k] = run.hi;
-            sh_lo[tid][k] = run.lo;
-            run = df_add(run, x);
-        }
-    }
-    __syncthreads();
-
-    // pass 2: re-merge and write prefix entries P[b] = sum of bins < b
-    df run[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) run[c] = {sh_hi[c][tid], sh_lo[c][tid]};
-#pragma unroll
-    for (int X = 0; X < TCW_MAX_DETECTORS; X++) cur[X] = cur0[X];
-    double *P = prefix + td.prefix_off;
-    for (uint32_t b = b0; b < b1; b++) {
-        float v[NC];
-        merge_bin(b, v);
+        // thread total (bins beyond N hold zeros)
+        df x[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            P[(int64_t)b * NC + c] = df_value(run[c]);
-            run[c] = df_add_f(run[c], v[c]);
-        }
-    }
-    if (b1 == N && b0 < b1) {
+            x[c] = df_from(v[0][c]);
 #pragma unroll
-        for (int c = 0; c < NC; c++) P[(int64_t)N * NC + c] = df_value(run[c]);
+            for (int j = 1; j < STAGE_BINS; j++) x[c] = df_add(x[c], df_from(v[j][c]));
+        }
+        // inclusive scan of thread totals within the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const df y = shfl_up_df(x[c], d);
+                if (lane >= d) x[c] = df_add(y, x[c]);
+            }
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                sh_hi[c][wave] = x[c].hi;
+                sh_lo[c][wave] = x[c].lo;
+            }
+        }
+        __syncthreads();
+        if (tid < NC) {  // exclusive scan of the wave totals, seeded by the carry
+            df run = {sh_hi[tid][STAGE_WG / 64], sh_lo[tid][STAGE_WG / 64]};
+            for (int w = 0; w < STAGE_WG / 64; w++) {
+                const df t = {sh_hi[tid][w], sh_lo[tid][w]};
+                sh_hi[tid][w] = run.hi;
+                sh_lo[tid][w] = run.lo;
+                run = df_add(run, t);
+            }
+            sh_hi[tid][STAGE_WG / 64] = run.hi;
+            sh_lo[tid][STAGE_WG / 64] = run.lo;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            df ex = shfl_up_df(x[c], 1);  // exclusive within the wave
+            if (lane == 0) ex = {0.0f, 0.0f};
+            df run = df_add(df{sh_hi[c][wave], sh_lo[c][wave]}, ex);
+#pragma unroll
+            for (int j = 0; j < STAGE_BINS; j++) {
+                const uint32_t b = b0 + j;
+                if (b < N) P[(int64_t)b * NC + c] = df_value(run);
+                run = df_add(run, df_from(v[j][c]));
+            }
+        }
+        __syncthreads();  // offsets consumed before the next round rewrites them
     }
+    if (want_prefix && tid < NC)
+        P[(int64_t)N * NC + tid] = df_value(df{sh_hi[tid][STAGE_WG / 64], sh_lo[tid][STAGE_WG / 64]});
 }
 
 // ----------------------------------------------------------------------------
@@ -263,7 +272,9 @@ __global__ __launch_bounds__(WG) void stage_kernel(
 // ----------------------------------------------------------------------------
 
 // F from window sums: determinant, condition guard and normalisation in FP64
-// (rect.cu:98-118 / exp.cu:109-129; F = 2 when the guard fails).
+// (rect.cu:98-118 / exp.cu:109-129; F = 2 when the guard fails).  The final
+// quotient uses the hardware FP64 reciprocal: its error is far below the
+// float32 rounding of the stored F.
 __device__ __forceinline__ float cell_F(double A, double B, double C, double far,
                                         double fai, double fbr, double fbi) {
     const double D = fma(A, B, -(C * C));
@@ -271,9 +282,10 @@ __device__ __forceinline__ float cell_F(double A, double B, double C, double far
     const double p = fma(far, far, fai * fai);
     const double q = fma(fbr, fbr, fbi * fbi);
     const double r = fma(far, fbr, fai * fbi);
-    const double num = fma(B, p, fma(A, q, -2.0 * C * r));
+    const double num = fma(B, p, fma(A, q, (-2.0 * C) * r));
     const bool ok = (T > 0.0) && (T * T < QCOND * D);
-    return ok ? (float)(num / D) : 2.0f;
+    const float F = (float)(num * __builtin_amdgcn_rcp(D));
+    return ok ? F : 2.0f;
 }
 
 // uint32 index maths of rect.cu:21-31 (start) and rect.cu:59-69 (end).
@@ -295,6 +307,12 @@ __device__ __forceinline__ uint32_t idx_end_q(uint32_t q, uint32_t N) {
 // ----------------------------------------------------------------------------
 // fused reductions
 // ----------------------------------------------------------------------------
+
+constexpr float LOG2E = 1.4426950408889634f;
+
+__device__ __forceinline__ float exp_rel(float x, float ref) {
+    return __builtin_amdgcn_exp2f((x - ref) * LOG2E);  // exp(x - ref); exp(-inf) = 0
+}
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -324,13 +342,55 @@ __device__ __forceinline__ void wave_argmax(float &bv, int &bi) {
     }
 }
 
-// Per-workgroup reduction state for one work item.  Each lane owns 4 columns;
-// each sub-tile brings R rows (wave-uniform) x those 4 columns.
+constexpr int MAX_RB = 512;
+constexpr float REF_SLACK = 60.0f;  // e^60 * cells per item stays far below FLT_MAX
+
+// Wave sums of R per-lane values.  Returns the sum of one of them, and in
+// `row` which one (-1: this lane holds none).  R = 1: a plain butterfly, lane
+// 0 holds the result.  R = 4: two halving exchange steps first (lanes pair up
+// and keep half of the rows each), then a butterfly on one value per lane;
+// row r ends up on lane 2*(r&1) + (r>>1) -- 7 shuffles instead of 24.
+template <int R>
+__device__ __forceinline__ float rows_sum(const float (&v)[R], int lane, int &row);
+
+template <>
+__device__ __forceinline__ float rows_sum<1>(const float (&v)[1], int lane, int &row) {
+    row = lane == 0 ? 0 : -1;
+    return wave_sum(v[0]);
+}
+
+template <>
+__device__ __forceinline__ float rows_sum<4>(const float (&v)[4], int lane, int &row) {
+    const bool b = lane & 1;
+    // step 1: b = 0 keeps rows {0,1}, b = 1 keeps rows {2,3}
+    const float s0 = b ? v[0] : v[2];
+    const float s1 = b ? v[1] : v[3];
+    const float r0 = __shfl_xor(s0, 1);
+    const float r1 = __shfl_xor(s1, 1);
+    const float u0 = (b ? v[2] : v[0]) + r0;
+    const float u1 = (b ? v[3] : v[1]) + r1;
+    // step 2: c = 0 keeps the first of the pair, c = 1 the second
+    const bool c = lane & 2;
+    const float z = (c ? u1 : u0) + __shfl_xor(c ? u0 : u1, 2);
+    float t = z;
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) t += __shfl_xor(t, o);
+    row = lane < 4 ? 2 * (b ? 1 : 0) + (c ? 1 : 0) : -1;
+    return t;
+}
+
+// Per-work-item reduction state.  A lane owns 4 columns; every sub-tile brings
+// R wave-uniform rows x those 4 columns, with invalid cells set to -inf.
+//  - argmax: lane maximum of the sub-tile; only when it beats the lane's best
+//    is the first (row-major) position searched;
+//  - BtSG: sum of exp(F - Mw), Mw the wave's running maximum (accumulators
+//    rescaled when it grows); column sums stay in registers across sub-tiles,
+//    row sums are wave-reduced and parked in LDS with their Mw.
 template <bool BTSG>
 struct Reducer {
     float best;
     int bidx;
-    float Mw;  // wave-uniform reference for the exp sums
+    float Mw;
     float Stot;
     float Scol[4];
 
@@ -343,58 +403,77 @@ struct Reducer {
         for (int c = 0; c < 4; c++) Scol[c] = 0.0f;
     }
 
-    // F[r][c]; rowm[r] row index (or -1 when the row is past the map);
-    // coln[c] column index (or -1).  Writes the row partials.
+    // F[r][c]: cell values (-inf when invalid); row index m0 + r; colidx[c]
+    // column index of the lane's c-th column; rl0: item-local row of r = 0.
     template <int R>
-    __device__ void add(const float (&F)[R][4], const int (&rowm)[R], const int (&coln)[4],
-                        uint32_t N_tau, int lane, float2 *rowpart_t, uint32_t n_colblk,
-                        int cb) {
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (rowm[r] >= 0 && coln[c] >= 0)
-                    argmax_merge(best, bidx, F[r][c], rowm[r] * (int)N_tau + coln[c]);
-        if (!BTSG) return;
+    __device__ void add(const float (&F)[R][4], int m0, const int (&colidx)[4], int N_tau,
+                        int lane, int rl0, float *sh_rowM, float *sh_rowS) {
         float lm = -INFINITY;
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (rowm[r] >= 0 && coln[c] >= 0) lm = fmaxf(lm, F[r][c]);
-        const float wm = wave_max(lm);
-        if (wm == -INFINITY) return;
-        if (wm > Mw) {
-            const float sc = (Mw == -INFINITY) ? 0.0f : __expf(Mw - wm);
+            for (int c = 0; c < 4; c++) lm = fmaxf(lm, F[r][c]);
+        if (lm > best) {
+            int rr = 0, cc = 0;
+            bool found = false;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (!found && F[r][c] == lm) {
+                        found = true;
+                        rr = r;
+                        cc = c;
+                    }
+            best = lm;
+            int col = colidx[0];
+#pragma unroll
+            for (int c = 1; c < 4; c++)
+                if (cc == c) col = colidx[c];
+            bidx = (m0 + rr) * N_tau + col;
+        }
+        if (!BTSG) return;
+        // exp(F - Mw) only has to stay finite, so the wave reference Mw is
+        // raised (wave maximum, accumulators rescaled) only when some lane
+        // sees a value more than REF_SLACK above it
+        if (__any(lm > Mw + REF_SLACK)) {
+            const float wm = wave_max(lm);
+            const float sc = exp_rel(Mw, wm);
             Stot *= sc;
 #pragma unroll
             for (int c = 0; c < 4; c++) Scol[c] *= sc;
             Mw = wm;
         }
+        if (Mw == -INFINITY) return;  // wave-uniform: no valid cell yet
+        float sr[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            float sr = 0.0f;
+            sr[r] = 0.0f;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const float e = (rowm[r] >= 0 && coln[c] >= 0) ? __expf(F[r][c] - Mw) : 0.0f;
-                sr += e;
+                const float e = exp_rel(F[r][c], Mw);
+                sr[r] += e;
                 Scol[c] += e;
             }
-            Stot += sr;
-            const float srw = wave_sum(sr);
-            if (lane == 0 && rowm[r] >= 0)
-                rowpart_t[(int64_t)rowm[r] * n_colblk + cb] = make_float2(Mw, srw);
+            Stot += sr[r];
+        }
+        int row;
+        const float srw = rows_sum<R>(sr, lane, row);
+        if (row >= 0) {
+            sh_rowM[rl0 + row] = Mw;
+            sh_rowS[rl0 + row] = srw;
         }
     }
 };
 
-// End of a work item: combine the 4 waves; write the item partial and the
-// column partials.  colmap(lane, c) gives the local column (0..255) of the
-// lane's c-th column.
+// End of a work item: combine the 4 waves, then write the item partial
+// {max, argmax, M, S} and, for BtSG, the row and column partial sums rescaled
+// to the item maximum M (so each is a single float).
 template <bool BTSG, typename ColMap>
 __device__ void finish_item(Reducer<BTSG> &red, int wave, int lane, const TmplDev &td,
-                            const ItemDev &it, int64_t item_id, ItemPart *ipart,
-                            float2 *colpart, ColMap colmap) {
+                            const ItemDev &it, int64_t item_id, int nrows_item,
+                            ItemPart *ipart, float *rowpart, float *colpart,
+                            const float *sh_rowM, const float *sh_rowS, ColMap colmap) {
     __shared__ float sh_best[4];
     __shared__ int sh_bidx[4];
     __shared__ float sh_M[4];
@@ -416,33 +495,32 @@ __device__ void finish_item(Reducer<BTSG> &red, int wave, int lane, const TmplDe
     }
     __syncthreads();
     const int tid = threadIdx.x;
+    const float M = fmaxf(fmaxf(sh_M[0], sh_M[1]), fmaxf(sh_M[2], sh_M[3]));
     if (BTSG) {
         const int n = it.cb * TILE_COLS + tid;
         if (n < (int)td.N_tau) {
-            float M = -INFINITY;
-#pragma unroll
-            for (int w = 0; w < 4; w++) M = fmaxf(M, sh_M[w]);
             float S = 0.0f;
 #pragma unroll
             for (int w = 0; w < 4; w++)
-                if (sh_M[w] != -INFINITY) S += sh_col[w][tid] * __expf(sh_M[w] - M);
-            colpart[td.colpart_off + (int64_t)n * td.n_rowblk + it.rb] = make_float2(M, S);
+                if (sh_M[w] != -INFINITY) S += sh_col[w][tid] * exp_rel(sh_M[w], M);
+            colpart[td.colpart_off + (int64_t)n * td.n_rowblk + it.rb] = S;
+        }
+        for (int rl = tid; rl < nrows_item; rl += WG) {
+            const int m = it.rb * td.rb_rows + rl;
+            rowpart[td.rowpart_off + (int64_t)m * td.n_colblk + it.cb] =
+                sh_rowS[rl] * exp_rel(sh_rowM[rl], M);
         }
     }
     if (tid == 0) {
         float bv = sh_best[0];
         int bi = sh_bidx[0];
-        float M = sh_M[0];
 #pragma unroll
-        for (int w = 1; w < 4; w++) {
-            argmax_merge(bv, bi, sh_best[w], sh_bidx[w]);
-            M = fmaxf(M, sh_M[w]);
-        }
+        for (int w = 1; w < 4; w++) argmax_merge(bv, bi, sh_best[w], sh_bidx[w]);
         float S = 0.0f;
         if (BTSG) {
 #pragma unroll
             for (int w = 0; w < 4; w++)
-                if (sh_M[w] != -INFINITY) S += sh_S[w] * __expf(sh_M[w] - M);
+                if (sh_M[w] != -INFINITY) S += sh_S[w] * exp_rel(sh_M[w], M);
         }
         ipart[item_id] = ItemPart{bv, bi, M, S};
     }
@@ -459,22 +537,48 @@ __device__ void finish_item(Reducer<BTSG> &red, int wave, int lane, const TmplDe
 // and j >= i always, so the incremental sum of rect.cu:75-91 equals
 // P[j] - P[i].  The item's j(D) range is staged in LDS (component-major,
 // padded every 4 entries so the 4-column lane stride hits distinct banks);
-// each lane computes a 4x4 block whose 16 cells touch 7 diagonals.
+// each lane computes a 4x4 block whose 16 cells touch 7 diagonals, loading
+// 4 entries for the first row and one more per following row.
 __host__ __device__ __forceinline__ int lds_pad(int d) { return d + (d >> 2); }
 
+// Work-item order.  Blocks go to the 8 XCDs round-robin (block b on XCD
+// b % 8) and every XCD has its own L2, so XCD x is given a contiguous run of
+// work items: the items of one template, which read the same prefix table,
+// then share one L2.
+#ifndef TCW_XCD_REMAP
+#define TCW_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int64_t block_slot() {
+#if TCW_XCD_REMAP
+    const int64_t G = gridDim.x, b = blockIdx.x;
+    const int64_t q = G >> 3, r = G & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+#else
+    return blockIdx.x;
+#endif
+}
+
+#ifndef TCW_FAST_MIN_WAVES
+#define TCW_FAST_MIN_WAVES 4
+#endif
+
 template <bool BTSG, bool STRICT>
-__global__ __launch_bounds__(WG) void rect_fast_kernel(
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(TCW_FAST_MIN_WAVES, 8)))
+void rect_fast_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
     const double *__restrict__ prefix, float *__restrict__ Fout,
-    ItemPart *__restrict__ ipart, float2 *__restrict__ rowpart,
-    float2 *__restrict__ colpart, int *__restrict__ flags) {
+    ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
+    float *__restrict__ colpart, int *__restrict__ flags) {
     extern __shared__ double lds[];
-    const int64_t item_id = item0 + blockIdx.x;
+    __shared__ float sh_rowM[MAX_RB];
+    __shared__ float sh_rowS[MAX_RB];
+    const int64_t item_id = item0 + block_slot();
     const ItemDev it = items[item_id];
     const TmplDev td = tds[it.t];
     const int RB = td.rb_rows;
     const int row0 = it.rb * RB;
     const int col0 = it.cb * TILE_COLS;
+    const int N_t0 = (int)td.N_t0, N_tau = (int)td.N_tau;
     const int nd = RB + TILE_COLS - 1;
     const int ndp = lds_pad(nd) + 1;
     const int D0 = row0 + col0;
@@ -496,39 +600,40 @@ __global__ __launch_bounds__(WG) void rect_fast_kernel(
     red.init();
     bool single = false;
     float *__restrict__ Ft = Fout + td.F_off;
-    float2 *rowpart_t = rowpart + td.rowpart_off;
-
-    int coln[4];
+    const int c0 = col0 + lane * 4;  // the lane's first column
+    const int nvc = min(max(N_tau - c0, 0), 4);
+    int colidx[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int n = col0 + lane * 4 + c;
-        coln[c] = n < (int)td.N_tau ? n : -1;
-    }
+    for (int c = 0; c < 4; c++) colidx[c] = c0 + c;
+    const bool cols_full = col0 + TILE_COLS <= N_tau;
 
     for (int s = 0; s < RB / 16; s++) {
         const int rl0 = s * 16 + wave * 4;
-        if (row0 + rl0 >= (int)td.N_t0) break;
-        const int dbase = rl0 + lane * 4;
+        const int m0 = row0 + rl0;
+        if (m0 >= N_t0) break;
+        const int nvr = min(N_t0 - m0, 4);
+        // lane's diagonals d = rl0 + 4*lane + x, x = 0..6; rl0 + 4*lane is a
+        // multiple of 4, so lds_pad(d) = lds_pad(rl0 + 4*lane) + x + x/4
+        const int lb = lds_pad(rl0 + lane * 4);
         double w[4][NC];
 #pragma unroll
         for (int c = 0; c < 4; c++)
 #pragma unroll
-            for (int k = 0; k < NC; k++) w[c][k] = lds[k * ndp + lds_pad(dbase + c)];
+            for (int k = 0; k < NC; k++) w[c][k] = lds[k * ndp + lb + c];
         float F[4][4];
-        int rowm[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int m = row0 + rl0 + r;
-            rowm[r] = m < (int)td.N_t0 ? m : -1;
             if (r > 0) {
 #pragma unroll
                 for (int c = 0; c < 3; c++)
 #pragma unroll
                     for (int k = 0; k < NC; k++) w[c][k] = w[c + 1][k];
+                const int x = r + 3;
 #pragma unroll
-                for (int k = 0; k < NC; k++) w[3][k] = lds[k * ndp + lds_pad(dbase + r + 3)];
+                for (int k = 0; k < NC; k++) w[3][k] = lds[k * ndp + lb + x + (x >> 2)];
             }
-            const int i = min(td.q0 + td.a * m, Nm1);
+            // window start: wave-uniform, read through the scalar cache
+            const int i = min(td.q0 + td.a * (m0 + r), Nm1);
             double Pi[NC];
 #pragma unroll
             for (int k = 0; k < NC; k++) Pi[k] = P[(int64_t)i * NC + k];
@@ -537,25 +642,37 @@ __global__ __launch_bounds__(WG) void rect_fast_kernel(
                 F[r][c] = cell_F(w[c][0] - Pi[0], w[c][1] - Pi[1], w[c][2] - Pi[2],
                                  w[c][3] - Pi[3], w[c][4] - Pi[4], w[c][5] - Pi[5],
                                  w[c][6] - Pi[6]);
-                if (STRICT && rowm[r] >= 0 && coln[c] >= 0) {
-                    int q = td.q00 + td.a * (m + coln[c]) - 1;
-                    q = min(max(q, 0), Nm1);
-                    single |= (q == i);
+                if (STRICT) {
+                    const int m = m0 + r;
+                    int64_t q = (int64_t)td.q00 + (int64_t)td.a * (m + colidx[c]) - 1;
+                    q = min(max(q, (int64_t)0), (int64_t)Nm1);
+                    single |= (q == i) && (r < nvr) && (c < nvc);
                 }
             }
-            if (rowm[r] >= 0) {
-                float *Frow = Ft + (int64_t)m * td.N_tau;
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-                    if (coln[c] >= 0) Frow[coln[c]] = F[r][c];
-            }
         }
-        red.template add<4>(F, rowm, coln, td.N_tau, lane, rowpart_t, td.n_colblk, it.cb);
+        if (cols_full && nvr == 4) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                float *Frow = Ft + (int64_t)(m0 + r) * N_tau + c0;
+#pragma unroll
+                for (int c = 0; c < 4; c++) Frow[c] = F[r][c];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const bool v = (r < nvr) && (c < nvc);
+                    if (v) Ft[(int64_t)(m0 + r) * N_tau + c0 + c] = F[r][c];
+                    F[r][c] = v ? F[r][c] : -INFINITY;
+                }
+        }
+        red.template add<4>(F, m0, colidx, N_tau, lane, rl0, sh_rowM, sh_rowS);
     }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     __syncthreads();
-    finish_item<BTSG>(red, wave, lane, td, it, item_id, ipart, colpart,
-                      [](int ln, int c) { return ln * 4 + c; });
+    finish_item<BTSG>(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
+                      colpart, sh_rowM, sh_rowS, [](int ln, int c) { return ln * 4 + c; });
 }
 
 // ----------------------------------------------------------------------------
@@ -592,14 +709,17 @@ template <bool BTSG, bool STRICT>
 __global__ __launch_bounds__(WG) void rect_generic_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
     const double *__restrict__ prefix, float *__restrict__ Fout,
-    ItemPart *__restrict__ ipart, float2 *__restrict__ rowpart,
-    float2 *__restrict__ colpart, int *__restrict__ flags) {
-    const int64_t item_id = item0 + blockIdx.x;
+    ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
+    float *__restrict__ colpart, int *__restrict__ flags) {
+    __shared__ float sh_rowM[MAX_RB];
+    __shared__ float sh_rowS[MAX_RB];
+    const int64_t item_id = item0 + block_slot();
     const ItemDev it = items[item_id];
     const TmplDev td = tds[it.t];
     const int RB = td.rb_rows;
     const int row0 = it.rb * RB;
     const int col0 = it.cb * TILE_COLS;
+    const int N_t0 = (int)td.N_t0, N_tau = (int)td.N_tau;
     const double *__restrict__ P = prefix + td.prefix_off;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -607,44 +727,40 @@ __global__ __launch_bounds__(WG) void rect_generic_kernel(
     red.init();
     bool single = false;
     float *__restrict__ Ft = Fout + td.F_off;
-    float2 *rowpart_t = rowpart + td.rowpart_off;
-
-    int coln[4];
+    int colidx[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int n = col0 + c * 64 + lane;
-        coln[c] = n < (int)td.N_tau ? n : -1;
-    }
+    for (int c = 0; c < 4; c++) colidx[c] = col0 + c * 64 + lane;
 
     for (int rl = wave; rl < RB; rl += 4) {
         const int m = row0 + rl;
-        if (m >= (int)td.N_t0) break;
+        if (m >= N_t0) break;
         const RowRect rr = rect_row(td, (uint32_t)m);
         double Pi[NC];
 #pragma unroll
         for (int k = 0; k < NC; k++) Pi[k] = P[(int64_t)rr.i0 * NC + k];
         float F[1][4];
-        int rowm[1] = {m};
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            const uint32_t n = coln[c] >= 0 ? (uint32_t)coln[c] : 0u;
+            const bool valid = colidx[c] < N_tau;
+            const uint32_t n = valid ? (uint32_t)colidx[c] : 0u;
             const uint32_t x = rr.x0 + n * td.dtau;
             const uint32_t i1 = idx_end_q(x / td.TAtom, td.N);
             // running maximum of i_t1 over n' <= n (the i_t1_last recursion)
             const uint32_t run = (n >= rr.nwrap) ? max(i1, rr.i1_pre) : i1;
             const uint32_t j = max(rr.i0, run + 1);
             const double *Pj = P + (int64_t)j * NC;
-            F[0][c] = cell_F(Pj[0] - Pi[0], Pj[1] - Pi[1], Pj[2] - Pi[2], Pj[3] - Pi[3],
-                             Pj[4] - Pi[4], Pj[5] - Pi[5], Pj[6] - Pi[6]);
-            if (STRICT && coln[c] >= 0) single |= (i1 == rr.i0);
-            if (coln[c] >= 0) Ft[(int64_t)m * td.N_tau + coln[c]] = F[0][c];
+            const float f = cell_F(Pj[0] - Pi[0], Pj[1] - Pi[1], Pj[2] - Pi[2], Pj[3] - Pi[3],
+                                   Pj[4] - Pi[4], Pj[5] - Pi[5], Pj[6] - Pi[6]);
+            if (STRICT) single |= valid && (i1 == rr.i0);
+            if (valid) Ft[(int64_t)m * N_tau + colidx[c]] = f;
+            F[0][c] = valid ? f : -INFINITY;
         }
-        red.template add<1>(F, rowm, coln, td.N_tau, lane, rowpart_t, td.n_colblk, it.cb);
+        red.template add<1>(F, m, colidx, N_tau, lane, rl, sh_rowM, sh_rowS);
     }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     __syncthreads();
-    finish_item<BTSG>(red, wave, lane, td, it, item_id, ipart, colpart,
-                      [](int ln, int c) { return c * 64 + ln; });
+    finish_item<BTSG>(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
+                      colpart, sh_rowM, sh_rowS, [](int ln, int c) { return c * 64 + ln; });
 }
 
 // Exponential window, direct weighted sum over the truncated support
@@ -658,14 +774,17 @@ template <bool BTSG, bool STRICT>
 __global__ __launch_bounds__(WG) void exp_direct_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
     const double *__restrict__ merged, float *__restrict__ Fout,
-    ItemPart *__restrict__ ipart, float2 *__restrict__ rowpart,
-    float2 *__restrict__ colpart, int *__restrict__ flags) {
-    const int64_t item_id = item0 + blockIdx.x;
+    ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
+    float *__restrict__ colpart, int *__restrict__ flags) {
+    __shared__ float sh_rowM[MAX_RB];
+    __shared__ float sh_rowS[MAX_RB];
+    const int64_t item_id = item0 + block_slot();
     const ItemDev it = items[item_id];
     const TmplDev td = tds[it.t];
     const int RB = td.rb_rows;
     const int row0 = it.rb * RB;
     const int col0 = it.cb * TILE_COLS;
+    const int N_t0 = (int)td.N_t0, N_tau = (int)td.N_tau;
     const double *__restrict__ X = merged + td.merged_off;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -673,19 +792,14 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
     red.init();
     bool single = false;
     float *__restrict__ Ft = Fout + td.F_off;
-    float2 *rowpart_t = rowpart + td.rowpart_off;
     const uint32_t TA = td.TAtom;
-
-    int coln[4];
+    int colidx[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int n = col0 + c * 64 + lane;
-        coln[c] = n < (int)td.N_tau ? n : -1;
-    }
+    for (int c = 0; c < 4; c++) colidx[c] = col0 + c * 64 + lane;
 
     for (int rl = wave; rl < RB; rl += 4) {
         const int m = row0 + rl;
-        if (m >= (int)td.N_t0) break;
+        if (m >= N_t0) break;
         const uint32_t t0m = td.t0 + (uint32_t)m * td.dt0;
         const uint32_t i0 = idx_start(t0m, td.t_min, TA, td.N);
         // first atom with t_i >= t0m (timestamps t_min + i*TA never wrap)
@@ -693,13 +807,13 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
         if (t0m > td.t_min) ilo = (uint32_t)(((uint64_t)(t0m - td.t_min) + TA - 1) / TA);
         const uint32_t lo = max(i0, ilo);
         float F[1][4];
-        int rowm[1] = {m};
         for (int c = 0; c < 4; c++) {
-            const uint32_t n = coln[c] >= 0 ? (uint32_t)coln[c] : 0u;
+            const bool valid = colidx[c] < N_tau;
+            const uint32_t n = valid ? (uint32_t)colidx[c] : 0u;
             const uint32_t tau = td.tau + n * td.dtau;
             const uint32_t t1 = t0m + 3u * tau;
             const uint32_t i1 = idx_end_q((uint32_t)(t1 - td.t_min + TA / 2) / TA, td.N);
-            if (STRICT && coln[c] >= 0) single |= (i1 == i0);
+            if (STRICT) single |= valid && (i1 == i0);
             // last atom with t_i <= t1
             int64_t ihi = -1;
             if (t1 >= td.t_min) ihi = (int64_t)((t1 - td.t_min) / TA);
@@ -713,7 +827,8 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
             // wave-uniform loop bounds: the longest lane decides
             int64_t hi_w = hi;
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) hi_w = max(hi_w, (int64_t)__shfl_xor((long long)hi_w, o));
+            for (int o = 32; o > 0; o >>= 1)
+                hi_w = max(hi_w, (int64_t)__shfl_xor((long long)hi_w, o));
             for (int64_t i = lo; i <= hi_w; i++) {
                 const double wi = (i <= hi) ? w : 0.0;
                 const double w2 = wi * wi;
@@ -727,15 +842,16 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
                 acc[6] = fma(xi[6], wi, acc[6]);
                 w *= rho;
             }
-            F[0][c] = cell_F(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6]);
-            if (coln[c] >= 0) Ft[(int64_t)m * td.N_tau + coln[c]] = F[0][c];
+            const float f = cell_F(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6]);
+            if (valid) Ft[(int64_t)m * N_tau + colidx[c]] = f;
+            F[0][c] = valid ? f : -INFINITY;
         }
-        red.template add<1>(F, rowm, coln, td.N_tau, lane, rowpart_t, td.n_colblk, it.cb);
+        red.template add<1>(F, m, colidx, N_tau, lane, rl, sh_rowM, sh_rowS);
     }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     __syncthreads();
-    finish_item<BTSG>(red, wave, lane, td, it, item_id, ipart, colpart,
-                      [](int ln, int c) { return c * 64 + ln; });
+    finish_item<BTSG>(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
+                      colpart, sh_rowM, sh_rowS, [](int ln, int c) { return c * 64 + ln; });
 }
 
 // ----------------------------------------------------------------------------
@@ -800,20 +916,21 @@ __device__ void block_argmax_d(double &bv, int &bi) {
     __syncthreads();
 }
 
+constexpr int FIN_SCALE_CAP = 4096;  // per-item scale factors kept in LDS
+
 template <bool BTSG>
 __global__ __launch_bounds__(WG) void finalize_kernel(
     const TmplDev *__restrict__ tds, const ItemPart *__restrict__ ipart,
-    const float2 *__restrict__ rowpart, const float2 *__restrict__ colpart,
+    const float *__restrict__ rowpart, const float *__restrict__ colpart,
     const int *__restrict__ flags, tcw_stats *__restrict__ stats) {
+    __shared__ float sh_scale[FIN_SCALE_CAP];
     const int t = blockIdx.x;
     const TmplDev td = tds[t];
     const int64_t nitems = (int64_t)td.n_rowblk * td.n_colblk;
+    const ItemPart *ip = ipart + td.item_begin;
     float bv = -INFINITY;
     int bi = 0x7fffffff;
-    for (int64_t k = threadIdx.x; k < nitems; k += WG) {
-        const ItemPart p = ipart[td.item_begin + k];
-        argmax_merge(bv, bi, p.maxF, p.idx);
-    }
+    for (int64_t k = threadIdx.x; k < nitems; k += WG) argmax_merge(bv, bi, ip[k].maxF, ip[k].idx);
     block_argmax(bv, bi);
     const float maxF = bv;
     const int64_t m_ML = bi / (int)td.N_tau;
@@ -822,20 +939,26 @@ __global__ __launch_bounds__(WG) void finalize_kernel(
     double lnB = NAN, t0_MP = NAN, tau_MP = NAN;
     int64_t m_MP = -1, n_MP = -1;
     if (BTSG) {
-        const double ref = (double)maxF;
+        // item k's partials are relative to its maximum M_k <= maxF
         double s = 0.0;
         for (int64_t k = threadIdx.x; k < nitems; k += WG) {
-            const ItemPart p = ipart[td.item_begin + k];
-            s += (double)p.S * exp((double)p.M - ref);
+            const float sc = exp_rel(ip[k].M, maxF);
+            if (k < FIN_SCALE_CAP) sh_scale[k] = sc;
+            s += (double)ip[k].S * exp((double)ip[k].M - (double)maxF);
         }
-        s = block_sum_d(s);
-        lnB = log(70.0 / ((double)td.N_t0 * (double)td.N_tau)) + ref + log(s);
+        s = block_sum_d(s);  // contains a barrier: sh_scale is complete
+        lnB = log(70.0 / ((double)td.N_t0 * (double)td.N_tau)) + (double)maxF + log(s);
+        auto scale = [&](int64_t k) -> float {
+            return k < FIN_SCALE_CAP ? sh_scale[k] : exp_rel(ip[k].M, maxF);
+        };
         double rv = -1.0;
         int ri = 0x7fffffff;
         for (int m = threadIdx.x; m < (int)td.N_t0; m += WG) {
+            const int rb = m / td.rb_rows;
+            const float *rp = rowpart + td.rowpart_off + (int64_t)m * td.n_colblk;
             double R = 0.0;
-            const float2 *rp = rowpart + td.rowpart_off + (int64_t)m * td.n_colblk;
-            for (uint32_t b = 0; b < td.n_colblk; b++) R += (double)rp[b].y * exp((double)rp[b].x - ref);
+            for (uint32_t cb = 0; cb < td.n_colblk; cb++)
+                R += (double)(rp[cb] * scale((int64_t)rb * td.n_colblk + cb));
             if (R > rv || (R == rv && m < ri)) {
                 rv = R;
                 ri = m;
@@ -846,9 +969,11 @@ __global__ __launch_bounds__(WG) void finalize_kernel(
         double cv = -1.0;
         int ci = 0x7fffffff;
         for (int n = threadIdx.x; n < (int)td.N_tau; n += WG) {
+            const int cb = n / TILE_COLS;
+            const float *cp = colpart + td.colpart_off + (int64_t)n * td.n_rowblk;
             double Cs = 0.0;
-            const float2 *cp = colpart + td.colpart_off + (int64_t)n * td.n_rowblk;
-            for (uint32_t b = 0; b < td.n_rowblk; b++) Cs += (double)cp[b].y * exp((double)cp[b].x - ref);
+            for (uint32_t rb = 0; rb < td.n_rowblk; rb++)
+                Cs += (double)(cp[rb] * scale((int64_t)rb * td.n_colblk + cb));
             if (Cs > cv || (Cs == cv && n < ci)) {
                 cv = Cs;
                 ci = n;
@@ -1112,8 +1237,8 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     if ((rc = p.d_prefix.ensure(sizeof(double) * std::max<int64_t>(p.prefix_doubles, 1)))) return rc;
     if ((rc = p.d_merged.ensure(sizeof(double) * std::max<int64_t>(p.merged_doubles, 1)))) return rc;
     if ((rc = p.d_ipart.ensure(sizeof(ItemPart) * std::max<size_t>(p.items.size(), 1)))) return rc;
-    if ((rc = p.d_rowpart.ensure(sizeof(float2) * std::max<int64_t>(p.rowpart_n, 1)))) return rc;
-    if ((rc = p.d_colpart.ensure(sizeof(float2) * std::max<int64_t>(p.colpart_n, 1)))) return rc;
+    if ((rc = p.d_rowpart.ensure(sizeof(float) * std::max<int64_t>(p.rowpart_n, 1)))) return rc;
+    if ((rc = p.d_colpart.ensure(sizeof(float) * std::max<int64_t>(p.colpart_n, 1)))) return rc;
     if ((rc = p.d_flags.ensure(sizeof(int) * std::max<int64_t>(p.T, 1)))) return rc;
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
@@ -1171,8 +1296,8 @@ void launch_maps(tcw_plan &p, float *F, hipStream_t st) {
     auto *prefix = (const double *)p.d_prefix.p;
     auto *merged = (const double *)p.d_merged.p;
     auto *ipart = (ItemPart *)p.d_ipart.p;
-    auto *rowpart = (float2 *)p.d_rowpart.p;
-    auto *colpart = (float2 *)p.d_colpart.p;
+    auto *rowpart = (float *)p.d_rowpart.p;
+    auto *colpart = (float *)p.d_colpart.p;
     auto *flags = (int *)p.d_flags.p;
     int64_t n;
     if ((n = p.mode_begin[MODE_RECT_FAST + 1] - p.mode_begin[MODE_RECT_FAST]) > 0)
@@ -1202,7 +1327,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)((p.T + 255) / 256)), dim3(256), 0, st,
                        flags, p.T);
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[0], st));
-    hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(WG), 0, st,
+    hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(STAGE_WG), 0, st,
                        (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
                        cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[1], st));
@@ -1214,7 +1339,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
         if (ev) TCW_HIP_TRY(hipEventRecord(ev[2], st));
         hipLaunchKernelGGL(finalize_kernel<true>, dim3((unsigned)p.T), dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
-                           (const float2 *)p.d_rowpart.p, (const float2 *)p.d_colpart.p,
+                           (const float *)p.d_rowpart.p, (const float *)p.d_colpart.p,
                            (const int *)p.d_flags.p, stats_dev);
     } else {
         if (strict)
@@ -1224,7 +1349,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
         if (ev) TCW_HIP_TRY(hipEventRecord(ev[2], st));
         hipLaunchKernelGGL(finalize_kernel<false>, dim3((unsigned)p.T), dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
-                           (const float2 *)p.d_rowpart.p, (const float2 *)p.d_colpart.p,
+                           (const float *)p.d_rowpart.p, (const float *)p.d_colpart.p,
                            (const int *)p.d_flags.p, stats_dev);
     }
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[3], st));
