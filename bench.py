@@ -42,12 +42,41 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+WORKLOADS = {
+    "rect_batched": "rect window, 4096 templates, N_sft~U[1000,8000], 10% gaps, "
+                    "N_t0=N_tau=N_sft//4",
+    "rect_plain": "rect window, 1 template, 500 SFTs (10% gaps), 500 t0 x 500 tau",
+    "exp_plain": "exp window, 1 template, 2 detectors x 4000 SFTs (15% gaps), "
+                 "2000 t0 x 2000 tau",
+    "exp_shared": "exp window, 256 templates x 8192 shared timestamps (2 detectors, 10% "
+                  "gaps), 1024 t0 x 512 tau",
+}
+
+
 def make_workload(args):
+    """(BatchedTransientMaps constructor kwargs builder, batch-like object)."""
     from pyfstat_amd import synth
 
     if args.workload == "rect_batched":
         return synth.workload_rect_batched(T=args.templates)
+    if args.workload == "rect_plain":
+        multi, win = synth.workload_rect_plain()
+        return ([multi], [win])
+    if args.workload == "exp_plain":
+        multi, win = synth.workload_exp_plain()
+        return ([multi], [win])
+    if args.workload == "exp_shared":
+        multis, win = synth.workload_exp_shared()
+        return (multis, [win] * len(multis))
     raise ValueError(args.workload)
+
+
+def build_batch(work, BtSG):
+    from pyfstat_amd.batched import BatchedTransientMaps
+
+    if isinstance(work, tuple):
+        return BatchedTransientMaps.from_multi_atoms(work[0], work[1], BtSG=BtSG)
+    return BatchedTransientMaps.from_synthetic(work, BtSG=BtSG)
 
 
 def plain_loop(b, batch, idx, BtSG):
@@ -180,7 +209,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="rect_batched", choices=["rect_batched"])
+    ap.add_argument("--workload", default="rect_batched", choices=sorted(WORKLOADS))
     ap.add_argument("--templates", type=int, default=4096)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-btsg", action="store_true")
@@ -211,14 +240,14 @@ def main():
     features, _ = tcw.init_transient_fstat_map_features("hip")
     log("generating workload ...")
     t_gen = time.perf_counter()
-    batch = make_workload(args)
-    log(f"workload ready in {time.perf_counter() - t_gen:.1f} s: T={batch.T}, "
-        f"atoms={batch.ts.size}, cells={int(batch.cells().sum()):.3e}")
+    work = make_workload(args)
     BtSG = not args.no_btsg
     use_graph = not args.no_graph
-    b = BatchedTransientMaps.from_synthetic(batch, BtSG=BtSG)
+    b = build_batch(work, BtSG)
+    batch = work if not isinstance(work, tuple) else None
     cells = b.total_cells
-    atoms_binned = int(batch.n_sft.sum())
+    st0 = None
+    log(f"workload ready in {time.perf_counter() - t_gen:.1f} s: T={b.T}, cells={cells:.3e}")
     log(f"plan: {cells} cells, scratch {b.scratch_bytes() / 2**30:.2f} GiB")
 
     if args.profile_mode:
@@ -259,12 +288,13 @@ def main():
     stage_ms /= reps
     log(f"stage/maps/finalize ms: {stage_ms.round(4).tolist()}")
 
-    parity = parity_sample(b, batch, args.check_templates)
+    atoms_binned = int(b.stats()["numAtoms"].sum())
+    parity = parity_sample(b, batch, args.check_templates) if batch is not None else None
     if parity:
         log(f"parity: {parity}")
 
     plain = None
-    if args.plain_templates > 0:
+    if args.plain_templates > 0 and batch is not None:
         rng = np.random.default_rng(9)
         idx = sorted(rng.choice(batch.T, size=min(args.plain_templates, batch.T),
                                 replace=False).tolist())
@@ -281,7 +311,7 @@ def main():
         log(f"plain: {plain}")
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and batch is not None:
         log("cpu baseline ...")
         cpu = cpu_baseline(batch, args.cpu_seconds)
         log(f"cpu: {cpu}")
@@ -307,9 +337,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "rect_batched: 4096 templates, N_sft~U[1000,8000], 10% gaps, "
-                        "N_t0=N_tau=N_sft//4, rect window, BtSG",
-            "templates": batch.T,
+            "workload": f"{args.workload}: {WORKLOADS[args.workload]}",
+            "templates": b.T,
             "cells": cells,
             "binned_atoms": atoms_binned,
             "BtSG": BtSG,
@@ -319,7 +348,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "rect_fast_kernel (all map work items of the step)",
+            "kernel": "map kernels of the step (rect_fast_kernel / exp_fast_kernel)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -48,11 +48,12 @@ constexpr double QCOND = (COND_MAX + 1.0) * (COND_MAX + 1.0) / COND_MAX;
 enum Mode : int32_t {
     MODE_RECT_FAST = 0,     // regular grid: dt0 == dtau == a*TAtom, no wrap-around
     MODE_RECT_GENERIC = 1,  // any rectangular window range, exact uint32 semantics
-    MODE_EXP_DIRECT = 2,    // exponential window, direct weighted sum
-    MODE_COUNT = 3
+    MODE_EXP_DIRECT = 2,    // exponential window, direct weighted sum (any grid)
+    MODE_EXP_FAST = 3,      // exponential window on the atom grid: recurrence in t0
+    MODE_COUNT = 4
 };
 
-constexpr int RB_DEFAULT[MODE_COUNT] = {128, 16, 16};
+constexpr int RB_DEFAULT[MODE_COUNT] = {128, 16, 16, 128};
 
 struct TmplDev {
     int64_t seg_begin, seg_end;  // detector segments of this template
@@ -68,9 +69,12 @@ struct TmplDev {
     uint32_t n_rowblk, n_colblk;
     int32_t mode;
     int32_t rb_rows;             // rows per work item
-    int32_t q0, q00, a;          // MODE_RECT_FAST index maths
+    int32_t q0, q00, a;          // MODE_RECT_FAST index maths; a also MODE_EXP_FAST
+    int32_t k00, L0, Ls, H;      // MODE_EXP_FAST: first start atom, window lengths
+                                 // L(n) = L0 + Ls*n atoms, atoms [0, H) reachable
+    int32_t n_seg, pad_seg;      // MODE_EXP_FAST: row chunks walked (>= n_rowblk)
+    int64_t etab_off;            // MODE_EXP_FAST: doubles, [n_seg][7][N_tau]
     uint32_t o_t0, o_t0Band, o_tau, o_tauBand;  // caller's window (posterior modes)
-    int32_t pad0, pad1;
 };
 
 struct SegDev {
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(STAGE_WG) void stage_kernel(
     const uint32_t *__restrict__ ts, const float *__restrict__ cols, int64_t cstride,
     double *__restrict__ prefix, double *__restrict__ merged) {
     const TmplDev td = tds[blockIdx.x];
-    const bool want_prefix = td.mode != MODE_EXP_DIRECT;
+    const bool want_prefix = td.mode == MODE_RECT_FAST || td.mode == MODE_RECT_GENERIC;
     const uint32_t N = td.N, TA = td.TAtom;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -386,13 +390,13 @@ __device__ __forceinline__ float rows_sum<4>(const float (&v)[4], int lane, int 
 //  - BtSG: sum of exp(F - Mw), Mw the wave's running maximum (accumulators
 //    rescaled when it grows); column sums stay in registers across sub-tiles,
 //    row sums are wave-reduced and parked in LDS with their Mw.
-template <bool BTSG>
+template <bool BTSG, int NCOLS = 4>
 struct Reducer {
     float best;
     int bidx;
     float Mw;
     float Stot;
-    float Scol[4];
+    float Scol[NCOLS];
 
     __device__ void init() {
         best = -INFINITY;
@@ -400,37 +404,36 @@ struct Reducer {
         Mw = -INFINITY;
         Stot = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 4; c++) Scol[c] = 0.0f;
+        for (int c = 0; c < NCOLS; c++) Scol[c] = 0.0f;
     }
 
     // F[r][c]: cell values (-inf when invalid); row index m0 + r; colidx[c]
     // column index of the lane's c-th column; rl0: item-local row of r = 0.
     template <int R>
-    __device__ void add(const float (&F)[R][4], int m0, const int (&colidx)[4], int N_tau,
+    __device__ void add(const float (&F)[R][NCOLS], int m0, const int (&colidx)[NCOLS], int N_tau,
                         int lane, int rl0, float *sh_rowM, float *sh_rowS) {
         float lm = -INFINITY;
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) lm = fmaxf(lm, F[r][c]);
-        if (lm > best) {
+            for (int c = 0; c < NCOLS; c++) lm = fmaxf(lm, F[r][c]);
+        if (lm > best || (lm == best && lm != -INFINITY)) {
             int rr = 0, cc = 0;
             bool found = false;
 #pragma unroll
             for (int r = 0; r < R; r++)
 #pragma unroll
-                for (int c = 0; c < 4; c++)
+                for (int c = 0; c < NCOLS; c++)
                     if (!found && F[r][c] == lm) {
                         found = true;
                         rr = r;
                         cc = c;
                     }
-            best = lm;
             int col = colidx[0];
 #pragma unroll
-            for (int c = 1; c < 4; c++)
+            for (int c = 1; c < NCOLS; c++)
                 if (cc == c) col = colidx[c];
-            bidx = (m0 + rr) * N_tau + col;
+            argmax_merge(best, bidx, lm, (m0 + rr) * N_tau + col);
         }
         if (!BTSG) return;
         // exp(F - Mw) only has to stay finite, so the wave reference Mw is
@@ -441,7 +444,7 @@ struct Reducer {
             const float sc = exp_rel(Mw, wm);
             Stot *= sc;
 #pragma unroll
-            for (int c = 0; c < 4; c++) Scol[c] *= sc;
+            for (int c = 0; c < NCOLS; c++) Scol[c] *= sc;
             Mw = wm;
         }
         if (Mw == -INFINITY) return;  // wave-uniform: no valid cell yet
@@ -450,7 +453,7 @@ struct Reducer {
         for (int r = 0; r < R; r++) {
             sr[r] = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
+            for (int c = 0; c < NCOLS; c++) {
                 const float e = exp_rel(F[r][c], Mw);
                 sr[r] += e;
                 Scol[c] += e;
@@ -469,8 +472,8 @@ struct Reducer {
 // End of a work item: combine the 4 waves, then write the item partial
 // {max, argmax, M, S} and, for BtSG, the row and column partial sums rescaled
 // to the item maximum M (so each is a single float).
-template <bool BTSG, typename ColMap>
-__device__ void finish_item(Reducer<BTSG> &red, int wave, int lane, const TmplDev &td,
+template <bool BTSG, int NCOLS, typename ColMap>
+__device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const TmplDev &td,
                             const ItemDev &it, int64_t item_id, int nrows_item,
                             ItemPart *ipart, float *rowpart, float *colpart,
                             const float *sh_rowM, const float *sh_rowS, ColMap colmap) {
@@ -491,7 +494,7 @@ __device__ void finish_item(Reducer<BTSG> &red, int wave, int lane, const TmplDe
     }
     if (BTSG) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) sh_col[wave][colmap(lane, c)] = red.Scol[c];
+        for (int c = 0; c < NCOLS; c++) sh_col[wave][colmap(lane, c)] = red.Scol[c];
     }
     __syncthreads();
     const int tid = threadIdx.x;
@@ -671,7 +674,7 @@ void rect_fast_kernel(
     }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     __syncthreads();
-    finish_item<BTSG>(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
+    finish_item(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
                       colpart, sh_rowM, sh_rowS, [](int ln, int c) { return ln * 4 + c; });
 }
 
@@ -759,7 +762,7 @@ __global__ __launch_bounds__(WG) void rect_generic_kernel(
     }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     __syncthreads();
-    finish_item<BTSG>(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
+    finish_item(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
                       colpart, sh_rowM, sh_rowS, [](int ln, int c) { return c * 64 + ln; });
 }
 
@@ -850,8 +853,287 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
     }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     __syncthreads();
-    finish_item<BTSG>(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
+    finish_item(red, wave, lane, td, it, item_id, min(RB, N_t0 - row0), ipart, rowpart,
                       colpart, sh_rowM, sh_rowS, [](int ln, int c) { return c * 64 + ln; });
+}
+
+// ----------------------------------------------------------------------------
+// exponential window on the atom grid: sliding recurrence along t0
+// ----------------------------------------------------------------------------
+//
+// When t0 - t0_data, dt0, tau and dtau are whole multiples of TAtom (and
+// nothing wraps around 2^32), the window of cell (m, n) covers the atoms
+// k(m) = k00 + a*m .. k(m) + L(n) - 1, L(n) = 3*tau_n/TAtom, clipped at the
+// data end, and atom k(m) + l has weight rho^l, rho = exp(-TAtom/tau_n)
+// (exp.cu:53-90: t1 = t0 + 3 tau, so i_t1 = k + L - 1, and t0 <= t_i <= t1).
+// The weighted window sum E(k) = sum_{l < L} rho^l x_{k+l} slides by one atom
+// at a time:
+//     E(k - 1) = rho E(k) + x_{k-1} - rho^L x_{k-1+L}
+// (rho^2 and rho^2L = e^-6 for the a^2, b^2, ab columns; rho^L = e^-3), so a
+// cell costs O(1) instead of O(L).  This first-order linear recurrence runs
+// down the rows in chunks of RB rows, in parallel:
+//   exp_seg_kernel   Y(s): the recurrence across chunk s started from E = 0;
+//   exp_scan_kernel  per tau column, E at every chunk top from E = 0 at the
+//                    first atom no window reaches: E_top(s-1) = rho^(a RB)
+//                    E_top(s) + Y(s);
+//   exp_fast_kernel  each chunk walks its rows from E_top, evaluating F.
+// Chunk s spans rows s*RB .. (s+1)*RB - 1 (rows past N_t0 are walked but not
+// output).  Atoms entering and leaving the windows of a chunk are staged in
+// LDS; one thread per tau column.
+
+struct ExpCol {
+    double rho[2];   // rho^2 (a2, b2, ab), rho (Fa, Fb)
+    double rhoL[2];  // rho^(2L) = exp(-6), rho^L = exp(-3)
+    int L;
+};
+
+__device__ __forceinline__ ExpCol exp_col(const TmplDev &td, int n) {
+    ExpCol ec;
+    ec.L = td.L0 + td.Ls * n;
+    const double tau = (double)td.tau + (double)n * (double)td.dtau;
+    const double r1 = exp(-(double)td.TAtom / tau);
+    ec.rho[0] = r1 * r1;
+    ec.rho[1] = r1;
+    ec.rhoL[1] = exp(-3.0);
+    ec.rhoL[0] = exp(-6.0);
+    return ec;
+}
+
+constexpr int MAX_RB_EXPF = 128;
+// atoms staged in LDS, 7 doubles each: 1088 * 56 B = 59.5 KB, which with the
+// kernel's 4 KB of row partials stays within a workgroup's 64 KB
+constexpr int EXPF_LDS_CAP = 1088;
+
+// Atoms a walk from row r_top down to row r_bot touches: entering atoms
+// [k(r_bot), k(r_top)) and leaving atoms shifted by L(n), n in [col0, col_last].
+__host__ __device__ __forceinline__ int expf_stage_atoms(int a, int rows, int Ls, int cols) {
+    const int nhead = a * (rows - 1);
+    return nhead + nhead + Ls * (cols - 1);
+}
+
+struct ExpWalk {
+    const double *X;
+    double *lds;
+    int64_t head_lo, tail_lo;
+    int nhead, stride;  // LDS rows: [c][stride], head atoms then tail atoms
+    int H;
+    bool in_lds;
+};
+
+__device__ __forceinline__ ExpWalk expf_stage(const TmplDev &td, const double *X, double *lds,
+                                              int lds_atoms, int r_top, int r_bot, int col0,
+                                              int col_last) {
+    ExpWalk wk;
+    wk.X = X;
+    wk.lds = lds;
+    wk.H = td.H;
+    wk.head_lo = (int64_t)td.k00 + (int64_t)td.a * r_bot;
+    wk.nhead = td.a * (r_top - r_bot);
+    wk.tail_lo = wk.head_lo + td.L0 + (int64_t)td.Ls * col0;
+    const int ntail = wk.nhead + td.Ls * (col_last - col0);
+    wk.stride = wk.nhead + ntail;
+    wk.in_lds = wk.stride <= lds_atoms;  // block-uniform
+    if (wk.in_lds) {
+        for (int idx = threadIdx.x; idx < wk.stride * NC; idx += WG) {
+            const int j = idx / NC;
+            const int c = idx - j * NC;
+            const int64_t atom = j < wk.nhead ? wk.head_lo + j : wk.tail_lo + (j - wk.nhead);
+            lds[c * wk.stride + j] = atom < wk.H ? X[atom * MERGED_STRIDE + c] : 0.0;
+        }
+    }
+    __syncthreads();
+    return wk;
+}
+
+// One row step: k -> k - a (a atom steps of the recurrence).
+__device__ __forceinline__ void expf_step(const ExpWalk &wk, const ExpCol &ec, int a, int k,
+                                          double (&E)[NC]) {
+    for (int l = 1; l <= a; l++) {
+        const int64_t kk = (int64_t)k - l;  // entering atom (wave-uniform)
+        const int64_t kt = kk + ec.L;       // leaving atom
+        double xh[NC], xt[NC];
+        if (wk.in_lds) {
+            const int jh = (int)(kk - wk.head_lo);
+            const int jt = wk.nhead + (int)(kt - wk.tail_lo);
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                xh[c] = wk.lds[c * wk.stride + jh];
+                xt[c] = wk.lds[c * wk.stride + jt];
+            }
+        } else {
+            const double *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
+            const double *pt = wk.X + min(kt, (int64_t)wk.H - 1) * MERGED_STRIDE;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                xh[c] = kk < wk.H ? ph[c] : 0.0;
+                xt[c] = kt < wk.H ? pt[c] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int g = c < 3 ? 0 : 1;
+            E[c] = fma(-ec.rhoL[g], xt[c], fma(ec.rho[g], E[c], xh[c]));
+        }
+    }
+}
+
+// Etab layout per template: [segment][component][tau column]
+__device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, int s, int c) {
+    return etab + td.etab_off + ((int64_t)s * NC + c) * td.N_tau;
+}
+
+// Y(s), s >= 1: the recurrence across chunk s (from its top row down to the
+// top row of chunk s-1) started from zero.  Work items (t, s, cb).
+__global__ __launch_bounds__(WG) void exp_seg_kernel(
+    const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items,
+    const double *__restrict__ merged, double *__restrict__ etab, int lds_atoms) {
+    extern __shared__ double lds[];
+    const ItemDev it = items[block_slot()];
+    const TmplDev td = tds[it.t];
+    const int RB = td.rb_rows;
+    const int col0 = it.cb * TILE_COLS;
+    const int n = col0 + threadIdx.x;
+    const int N_tau = (int)td.N_tau;
+    const bool valid = n < N_tau;
+    const int col_last = min(col0 + TILE_COLS, N_tau) - 1;
+    const int r_top = (it.rb + 1) * RB - 1, r_bot = it.rb * RB - 1;
+    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_atoms, r_top, r_bot, col0,
+                                  col_last);
+    const ExpCol ec = exp_col(td, valid ? n : col0);
+    double E[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) E[c] = 0.0;
+    int k = td.k00 + td.a * r_top;
+    for (int m = r_top; m > r_bot; m--) {
+        expf_step(wk, ec, td.a, k, E);
+        k -= td.a;
+    }
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) expf_etab(etab, td, it.rb, c)[n] = E[c];
+    }
+}
+
+// Per tau column: E at every chunk top.  Work items (t, -, cb).
+__global__ __launch_bounds__(WG) void exp_scan_kernel(const TmplDev *__restrict__ tds,
+                                                      const ItemDev *__restrict__ items,
+                                                      double *__restrict__ etab) {
+    const ItemDev it = items[block_slot()];
+    const TmplDev td = tds[it.t];
+    const int n = it.cb * TILE_COLS + threadIdx.x;
+    if (n >= (int)td.N_tau) return;
+    const double tau = (double)td.tau + (double)n * (double)td.dtau;
+    const double steps = (double)td.a * td.rb_rows;
+    const double m1 = exp(-(double)td.TAtom * steps / tau);
+    const double mult[2] = {m1 * m1, m1};
+    double E[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) E[c] = 0.0;
+    for (int s = td.n_seg - 1; s >= 1; s--) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            double *slot = expf_etab(etab, td, s, c) + n;
+            const double Y = *slot;
+            *slot = E[c];
+            E[c] = fma(mult[c < 3 ? 0 : 1], E[c], Y);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) expf_etab(etab, td, 0, c)[n] = E[c];
+}
+
+// The map: chunk rb walks rows (rb+1)*RB - 1 .. rb*RB from E_top(rb).
+template <bool BTSG, bool STRICT>
+__global__ __launch_bounds__(WG) void exp_fast_kernel(
+    const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
+    const double *__restrict__ merged, const double *__restrict__ etab,
+    float *__restrict__ Fout, ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
+    float *__restrict__ colpart, int *__restrict__ flags, int lds_atoms) {
+    extern __shared__ double lds[];
+    __shared__ float sh_rowM[4][MAX_RB_EXPF];
+    __shared__ float sh_rowS[4][MAX_RB_EXPF];
+    __shared__ float sh_best[4];
+    __shared__ int sh_bidx[4];
+    __shared__ float sh_M[4];
+    __shared__ float sh_S[4];
+    const int64_t item_id = item0 + block_slot();
+    const ItemDev it = items[item_id];
+    const TmplDev td = tds[it.t];
+    const int RB = td.rb_rows;
+    const int row0 = it.rb * RB;
+    const int N_t0 = (int)td.N_t0, N_tau = (int)td.N_tau;
+    const int r_top = row0 + RB - 1;
+    const int m_hi = min(r_top, N_t0 - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int col0 = it.cb * TILE_COLS;
+    const int n = col0 + threadIdx.x;
+    const bool valid = n < N_tau;
+    const int col_last = min(col0 + TILE_COLS, N_tau) - 1;
+    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_atoms, r_top, row0, col0,
+                                  col_last);
+    const ExpCol ec = exp_col(td, valid ? n : col0);
+    double E[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) E[c] = valid ? expf_etab((double *)etab, td, it.rb, c)[n] : 0.0;
+
+    Reducer<BTSG, 1> red;
+    red.init();
+    bool single = false;
+    float *__restrict__ Ft = Fout + td.F_off;
+    const int colidx[1] = {n};
+    int k = td.k00 + td.a * r_top;
+    for (int m = r_top; m >= row0; m--) {
+        if (m <= m_hi) {  // wave-uniform: rows past the map are only walked
+            const float f = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
+            if (valid) Ft[(int64_t)m * N_tau + n] = f;
+            if (STRICT) single |= valid && (k >= (int)td.N - 1);
+            const float F1[1][1] = {{valid ? f : -INFINITY}};
+            red.template add<1>(F1, m, colidx, N_tau, lane, m - row0, sh_rowM[wave],
+                                sh_rowS[wave]);
+        }
+        if (m > row0) expf_step(wk, ec, td.a, k, E);
+        k -= td.a;
+    }
+    if (STRICT && single) atomicOr(flags + it.t, 1);
+    // combine: each column belongs to one lane; each row to all 4 waves
+    wave_argmax(red.best, red.bidx);
+    float stot = 0.0f;
+    if (BTSG) stot = wave_sum(red.Stot);
+    if (lane == 0) {
+        sh_best[wave] = red.best;
+        sh_bidx[wave] = red.bidx;
+        sh_M[wave] = red.Mw;
+        sh_S[wave] = stot;
+    }
+    __syncthreads();
+    const float M = fmaxf(fmaxf(sh_M[0], sh_M[1]), fmaxf(sh_M[2], sh_M[3]));
+    if (BTSG) {
+        if (valid)
+            colpart[td.colpart_off + (int64_t)n * td.n_rowblk + it.rb] =
+                red.Mw == -INFINITY ? 0.0f : red.Scol[0] * exp_rel(red.Mw, M);
+        const int nrows = m_hi - row0 + 1;
+        for (int rl = threadIdx.x; rl < nrows; rl += WG) {
+            float S = 0.0f;
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (sh_M[w] != -INFINITY) S += sh_rowS[w][rl] * exp_rel(sh_rowM[w][rl], M);
+            rowpart[td.rowpart_off + (int64_t)(row0 + rl) * td.n_colblk + it.cb] = S;
+        }
+    }
+    if (threadIdx.x == 0) {
+        float bv = sh_best[0];
+        int bi = sh_bidx[0];
+#pragma unroll
+        for (int w = 1; w < 4; w++) argmax_merge(bv, bi, sh_best[w], sh_bidx[w]);
+        float S = 0.0f;
+        if (BTSG) {
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (sh_M[w] != -INFINITY) S += sh_S[w] * exp_rel(sh_M[w], M);
+        }
+        ipart[item_id] = ItemPart{bv, bi, M, S};
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -1053,7 +1335,7 @@ struct DevBuf {
     }
 };
 
-int g_rb_override[MODE_COUNT] = {0, 0, 0};
+int g_rb_override[MODE_COUNT] = {0, 0, 0, 0};
 
 }  // namespace
 
@@ -1063,13 +1345,17 @@ struct tcw_plan {
     int64_t T = 0;
     std::vector<TmplDev> tmpl;
     std::vector<SegDev> segs;
-    std::vector<ItemDev> items;   // grouped by mode
-    int64_t mode_begin[MODE_COUNT + 1] = {0, 0, 0, 0};
+    std::vector<ItemDev> items;      // grouped by mode
+    std::vector<ItemDev> seg_items;  // MODE_EXP_FAST (t, chunk, column block), chunk >= 1
+    std::vector<ItemDev> col_items;  // MODE_EXP_FAST (t, -, column block)
+    int64_t mode_begin[MODE_COUNT + 1] = {0, 0, 0, 0, 0};
     int64_t prefix_doubles = 0, merged_doubles = 0, rowpart_n = 0, colpart_n = 0;
+    int64_t etab_doubles = 0;
     int64_t F_total = 0;
-    int max_lds_bytes = 0;
+    int max_lds_bytes = 0;  // rect_fast_kernel dynamic LDS
+    int max_lds_expf = 0;   // exp_fast_kernel dynamic LDS
     DevBuf d_tmpl, d_segs, d_items, d_prefix, d_merged, d_ipart, d_rowpart, d_colpart,
-        d_flags;
+        d_flags, d_seg_items, d_col_items, d_etab;
     hipGraphExec_t graph = nullptr;
     hipStream_t cap_stream = nullptr;  // private stream the graph is captured on
     const void *g_ts = nullptr, *g_cols = nullptr, *g_F = nullptr, *g_stats = nullptr;
@@ -1078,8 +1364,12 @@ struct tcw_plan {
         tmpl.clear();
         segs.clear();
         items.clear();
+        seg_items.clear();
+        col_items.clear();
         prefix_doubles = merged_doubles = rowpart_n = colpart_n = F_total = 0;
+        etab_doubles = 0;
         max_lds_bytes = 0;
+        max_lds_expf = 0;
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
     }
@@ -1095,6 +1385,9 @@ struct tcw_plan {
         d_rowpart.release();
         d_colpart.release();
         d_flags.release();
+        d_seg_items.release();
+        d_col_items.release();
+        d_etab.release();
     }
 };
 
@@ -1170,6 +1463,22 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
     int mode;
     if (w.type == TCW_TRANSIENT_EXPONENTIAL) {
         mode = MODE_EXP_DIRECT;
+        // on the atom grid and free of uint32 wrap-around: the recurrence path
+        const uint64_t end = (uint64_t)w.t0 + w.t0Band + 3ull * ((uint64_t)w.tau + w.tauBand) +
+                             TAtom;
+        const bool on_grid = w.t0 >= t_min && (w.t0 - t_min) % TAtom == 0 &&
+                             w.dt0 % TAtom == 0 && w.tau % TAtom == 0 &&
+                             w.dtau % TAtom == 0 && w.tau >= TAtom;
+        if (!(p.flags & TCW_FLAG_GENERIC) && on_grid && end < (1ull << 32)) {
+            mode = MODE_EXP_FAST;
+            td.k00 = (int32_t)((w.t0 - t_min) / TAtom);
+            td.a = (int32_t)(w.dt0 / TAtom);
+            td.L0 = (int32_t)(3ull * w.tau / TAtom);
+            td.Ls = (int32_t)(3ull * w.dtau / TAtom);
+            const int64_t kmax = (int64_t)td.k00 + (int64_t)td.a * (N_t0 - 1);
+            const int64_t Lmax = (int64_t)td.L0 + (int64_t)td.Ls * (N_tau - 1);
+            td.H = (int32_t)std::min<int64_t>((int64_t)N, kmax + Lmax);
+        }
     } else {
         mode = MODE_RECT_GENERIC;
         const uint64_t half = TAtom / 2;
@@ -1188,9 +1497,27 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
     td.rb_rows = rb;
     td.n_rowblk = (uint32_t)((N_t0 + rb - 1) / rb);
     td.n_colblk = (uint32_t)((N_tau + TILE_COLS - 1) / TILE_COLS);
-    if (mode == MODE_EXP_DIRECT) {
+    if (mode == MODE_EXP_DIRECT || mode == MODE_EXP_FAST) {
         td.merged_off = p.merged_doubles;
         p.merged_doubles += (int64_t)N * MERGED_STRIDE;
+        if (mode == MODE_EXP_FAST) {
+            // chunks until the top row's window starts at or beyond H (E = 0 there)
+            const int64_t steps = ((int64_t)td.H - td.k00 + td.a - 1) / td.a;  // rows to reach H
+            int64_t nseg = (std::max<int64_t>(steps, 0) + 1 + rb - 1) / rb;
+            nseg = std::max<int64_t>(nseg, (int64_t)td.n_rowblk);
+            nseg = std::max<int64_t>(nseg, 1);
+            td.n_seg = (int32_t)nseg;
+            td.etab_off = p.etab_doubles;
+            p.etab_doubles += nseg * NC * N_tau;
+            for (int64_t sg = 1; sg < nseg; sg++)
+                for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
+                    p.seg_items.push_back(ItemDev{(int32_t)t, (int32_t)sg, (int32_t)cbk, 0});
+            for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
+                p.col_items.push_back(ItemDev{(int32_t)t, 0, (int32_t)cbk, 0});
+            const int need = expf_stage_atoms(td.a, rb + 1, td.Ls, TILE_COLS);
+            if (need <= EXPF_LDS_CAP)
+                p.max_lds_expf = std::max(p.max_lds_expf, (int)(sizeof(double) * NC * need));
+        }
     } else {
         td.prefix_off = p.prefix_doubles;
         p.prefix_doubles += ((int64_t)N + 1) * NC;
@@ -1240,6 +1567,11 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     if ((rc = p.d_rowpart.ensure(sizeof(float) * std::max<int64_t>(p.rowpart_n, 1)))) return rc;
     if ((rc = p.d_colpart.ensure(sizeof(float) * std::max<int64_t>(p.colpart_n, 1)))) return rc;
     if ((rc = p.d_flags.ensure(sizeof(int) * std::max<int64_t>(p.T, 1)))) return rc;
+    if ((rc = p.d_seg_items.ensure(sizeof(ItemDev) * std::max<size_t>(p.seg_items.size(), 1))))
+        return rc;
+    if ((rc = p.d_col_items.ensure(sizeof(ItemDev) * std::max<size_t>(p.col_items.size(), 1))))
+        return rc;
+    if ((rc = p.d_etab.ensure(sizeof(double) * std::max<int64_t>(p.etab_doubles, 1)))) return rc;
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
         return async ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
@@ -1248,6 +1580,8 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     TCW_HIP_TRY(up(p.d_tmpl.p, p.tmpl.data(), sizeof(TmplDev) * p.tmpl.size()));
     TCW_HIP_TRY(up(p.d_segs.p, p.segs.data(), sizeof(SegDev) * p.segs.size()));
     TCW_HIP_TRY(up(p.d_items.p, p.items.data(), sizeof(ItemDev) * p.items.size()));
+    TCW_HIP_TRY(up(p.d_seg_items.p, p.seg_items.data(), sizeof(ItemDev) * p.seg_items.size()));
+    TCW_HIP_TRY(up(p.d_col_items.p, p.col_items.data(), sizeof(ItemDev) * p.col_items.size()));
     return TCW_OK;
 }
 
@@ -1308,6 +1642,11 @@ void launch_maps(tcw_plan &p, float *F, hipStream_t st) {
         hipLaunchKernelGGL((rect_generic_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG), 0,
                            st, tds, items, p.mode_begin[MODE_RECT_GENERIC], prefix, F, ipart,
                            rowpart, colpart, flags);
+    if ((n = p.mode_begin[MODE_EXP_FAST + 1] - p.mode_begin[MODE_EXP_FAST]) > 0)
+        hipLaunchKernelGGL((exp_fast_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG),
+                           p.max_lds_expf, st, tds, items, p.mode_begin[MODE_EXP_FAST], merged,
+                           (const double *)p.d_etab.p, F, ipart, rowpart, colpart, flags,
+                           p.max_lds_expf / (int)(sizeof(double) * NC));
     if ((n = p.mode_begin[MODE_EXP_DIRECT + 1] - p.mode_begin[MODE_EXP_DIRECT]) > 0)
         hipLaunchKernelGGL((exp_direct_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG), 0,
                            st, tds, items, p.mode_begin[MODE_EXP_DIRECT], merged, F, ipart,
@@ -1330,6 +1669,15 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(STAGE_WG), 0, st,
                        (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
                        cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
+    if (!p.seg_items.empty())
+        hipLaunchKernelGGL(exp_seg_kernel, dim3((unsigned)p.seg_items.size()), dim3(WG),
+                           p.max_lds_expf, st, (const TmplDev *)p.d_tmpl.p,
+                           (const ItemDev *)p.d_seg_items.p, (const double *)p.d_merged.p,
+                           (double *)p.d_etab.p, p.max_lds_expf / (int)(sizeof(double) * NC));
+    if (!p.col_items.empty())
+        hipLaunchKernelGGL(exp_scan_kernel, dim3((unsigned)p.col_items.size()), dim3(WG), 0, st,
+                           (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
+                           (double *)p.d_etab.p);
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[1], st));
     if (btsg) {
         if (strict)
@@ -1438,7 +1786,8 @@ int64_t tcw_plan_scratch_bytes(const tcw_plan *p) {
     if (!p) return 0;
     return (int64_t)(p->d_tmpl.cap + p->d_segs.cap + p->d_items.cap + p->d_prefix.cap +
                      p->d_merged.cap + p->d_ipart.cap + p->d_rowpart.cap + p->d_colpart.cap +
-                     p->d_flags.cap);
+                     p->d_flags.cap + p->d_seg_items.cap + p->d_col_items.cap +
+                     p->d_etab.cap);
 }
 
 int tcw_batch_run(tcw_handle *h, tcw_plan *p, const uint32_t *ts, const float *cols,
@@ -1569,8 +1918,10 @@ int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win, float 
 // Testing/tuning hook: rows per work item for a kernel mode (0 restores the default).
 int tcw_set_rows_per_item(int mode, int rows) {
     if (mode < 0 || mode >= MODE_COUNT) return fail(TCW_ERR_INVALID, "bad mode");
-    if (rows != 0 && (rows < 16 || rows > 512 || rows % 16))
-        return fail(TCW_ERR_INVALID, "rows per item must be a multiple of 16 in [16, 512]");
+    const int maxr = mode == MODE_EXP_FAST ? MAX_RB_EXPF : MAX_RB;
+    if (rows != 0 && (rows < 16 || rows > maxr || rows % 16))
+        return fail(TCW_ERR_INVALID, "rows per item must be a multiple of 16 in [16, " +
+                                         std::to_string(maxr) + "]");
     g_rb_override[mode] = rows;
     return TCW_OK;
 }

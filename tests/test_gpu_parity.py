@@ -147,6 +147,30 @@ def test_exp_window_variants(hip_ready, case):
     _check_full(multi, win)
 
 
+@pytest.mark.parametrize(
+    "case",
+    [
+        # on the atom grid: the recurrence kernel
+        dict(t0=0, t0Band=299 * 1800, dt0=1800, tau=3600, tauBand=199 * 1800, dtau=1800),
+        dict(t0=37 * 1800, t0Band=150 * 3600, dt0=3600, tau=1800, tauBand=90 * 5400, dtau=5400),
+        # rows past the end of the data, windows past it
+        dict(t0=500 * 1800, t0Band=200 * 1800, dt0=1800, tau=7200, tauBand=300 * 1800, dtau=1800),
+    ],
+)
+def test_exp_recurrence_vs_oracle_and_direct(hip_ready, case):
+    from pyfstat_amd import tcw_fstat_map_funcs as tcw
+
+    multi, _ = synth.workload_exp_plain(n_sft=700, seed_index=31)
+    c = dict(case)
+    c["t0"] = synth.T_START + c["t0"]
+    win = transientWindowRange_t(type=TRANSIENT_EXPONENTIAL, **c)
+    fm, om, errs = _check_full(multi, win)
+    fd = tcw.hip_compute_transient_fstat_map(multi, win, BtSG=True, force_generic=True)
+    rel = np.abs(fd.F_mn - fm.F_mn) / np.maximum(np.abs(fm.F_mn), 1e-30)
+    near = np.abs(om.cond / 1e4 - 1) < 1e-3
+    assert np.where(near, 0, rel).max() < 2e-5
+
+
 def test_merge_offgrid_and_duplicate_bins(hip_ready):
     """Timestamps off the TAtom grid, several atoms per bin, 3 detectors."""
     from pyfstat_amd.atoms import FstatAtomVector, MultiFstatAtomVector
