@@ -50,7 +50,13 @@ WORKLOADS = {
                  "2000 t0 x 2000 tau",
     "exp_shared": "exp window, 256 templates x 8192 shared timestamps (2 detectors, 10% "
                   "gaps), 1024 t0 x 512 tau",
+    "exp_refine": "exp-window refinement: 4096 templates x 2 detectors x 2048 SFTs (10% "
+                  "gaps), 256 t0 x 256 tau maps, 16 grid maxima each = 65536 candidates, "
+                  "20 ascent steps in tau on fixed supports",
 }
+
+REFINE_METRIC = "refined candidate-steps/s on 1 GPU (2F + d2F/dtau per step)"
+FP64_VALU_PEAK_TFLOPS = 78.6
 
 
 def make_workload(args):
@@ -204,6 +210,163 @@ def parity_sample(b, batch, n, seed=3):
     }
 
 
+def run_refine(args):
+    """The refinement workload: setup (maps, grid maxima, supports) untimed;
+    one step = tcw_refine_ascent over every candidate (n_steps proposals, each
+    a fused 2F + derivative evaluation on the candidate's support)."""
+    import torch
+
+    from oracle import refine_ref as rr
+    from pyfstat_amd import synth
+    from pyfstat_amd import tcw_fstat_map_funcs as tcw
+    from pyfstat_amd.batched import BatchedTransientMaps
+    from pyfstat_amd.refine import ExpRefineProblem, MergedAtoms, candidates_from_maps
+
+    tcw.init_transient_fstat_map_features("hip")
+    log("generating refinement workload ...")
+    t_gen = time.perf_counter()
+    work = synth.workload_exp_refine(T=args.templates)
+    b = BatchedTransientMaps(work.ts, work.cols, work.seg_off, work.seg_begin, work.windows,
+                             work.TAtom)
+    b.run()
+    ct, t0, tau, Fg = candidates_from_maps(b, per_template=16)
+    merged = MergedAtoms.from_batch(b)
+    b.close()
+    prob = ExpRefineProblem(merged, ct, t0, tau)
+    TA = work.TAtom
+    lo = torch.clamp(prob.tau_init - TA, min=float(TA))
+    hi = prob.tau_init + TA
+    L = prob.support_lengths
+    torch.cuda.synchronize()
+    log(f"ready in {time.perf_counter() - t_gen:.1f} s: {prob.n} candidates, mean support "
+        f"{L.mean():.1f} bins")
+    n_steps = args.refine_steps
+    for _ in range(args.warmup):
+        prob.ascent(merged.cols, tau_lo=lo, tau_hi=hi, n_steps=n_steps)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        tau_r, twoF, nacc = prob.ascent(merged.cols, tau_lo=lo, tau_hi=hi, n_steps=n_steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    evals = prob.n * (n_steps + 1)  # the first evaluation at the start point
+    log(f"ascent: {ms:.3f} ms per step of {prob.n} candidates x {n_steps} proposals")
+
+    # forward alone and forward+backward through autograd, for the record
+    cols = merged.cols
+    reps = 10
+    e0.record(stream)
+    for _ in range(reps):
+        prob.forward_raw(cols, prob.t0_init, prob.tau_init)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    c = cols.clone().requires_grad_(True)
+    tt = prob.tau_init.clone().requires_grad_(True)
+    e0.record(stream)
+    for _ in range(reps):
+        f2, lnb = prob(c, prob.t0_init, tt)
+        f2.sum().backward()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    fb_ms = e0.elapsed_time(e1) / reps
+
+    F_init = prob.forward_raw(cols, prob.t0_init, prob.tau_init, want_saved=False)[0]
+    gain = (twoF - F_init).cpu().numpy()
+    nacc_h = nacc.cpu().numpy()
+    # parity: a sample against the FP64 reference, and the ascent against a scan
+    rng = np.random.default_rng(11)
+    idx = np.sort(rng.choice(prob.n, size=min(args.check_templates * 4, prob.n), replace=False))
+    cols_h = cols.cpu().numpy().astype(np.float64)
+    t0_h = prob.t0_init.cpu().numpy()
+    tau_h = tau_r.cpu().numpy()
+    lo_h, hi_h = lo.cpu().numpy(), hi.cpu().numpy()
+    twoF_h = twoF.cpu().numpy()
+    worst, scan_gap = 0.0, 0.0
+    for i in idx:
+        t = int(prob.cand_tmpl[i])
+        sub = cols_h[:, merged.m_off[t]:merged.m_off[t + 1]]
+        F, _, _, _ = rr.evaluate(sub, float(merged.t_data[t]), TA, int(prob.k_lo[i]),
+                                 int(prob.k_hi[i]), float(t0_h[i]), float(tau_h[i]))
+        worst = max(worst, abs(twoF_h[i] - 2 * F) / abs(2 * F))
+        Fmax, _ = rr.scan_max_tau(sub, float(merged.t_data[t]), TA, int(prob.k_lo[i]),
+                                  int(prob.k_hi[i]), float(t0_h[i]), float(lo_h[i]),
+                                  float(hi_h[i]), n=401)
+        scan_gap = max(scan_gap, (2 * Fmax - twoF_h[i]) / abs(2 * Fmax))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the FP64 numpy reference, one core: evaluations (2F + derivatives) at
+        # the candidates' start points, repeated until about --cpu-seconds pass
+        sample = rng.choice(prob.n, size=min(4096, prob.n), replace=False)
+        done, bins = 0, 0
+        c0 = time.perf_counter()
+        for i in sample:
+            t = int(prob.cand_tmpl[i])
+            sub = cols_h[:, merged.m_off[t]:merged.m_off[t + 1]]
+            rr.evaluate(sub, float(merged.t_data[t]), TA, int(prob.k_lo[i]), int(prob.k_hi[i]),
+                        float(t0_h[i]), float(prob.tau_init[i]))
+            done += 1
+            bins += int(L[i])
+            if time.perf_counter() - c0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - c0
+        cpu = {"value": done / dt, "unit": "candidate-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{done} random candidates ({bins} support bins, {dt:.1f} s), FP64 "
+                         "numpy reference evaluate() (2F + all derivatives), single core"}
+        log(f"cpu: {cpu}")
+    flops = 42.0 * float(L.sum()) * (n_steps + 1)
+    achieved = flops / (ms / 1e3) / 1e12
+    out = {
+        "metric": REFINE_METRIC,
+        "value": evals / (ms / 1e3),
+        "unit": "candidate-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"exp_refine: {WORKLOADS['exp_refine']}",
+            "candidates": prob.n,
+            "ascent_steps": n_steps,
+            "mean_support_bins": float(L.mean()),
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "refine_ascent_kernel",
+            "achieved": achieved,
+            "peak": FP64_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_VALU_PEAK_TFLOPS,
+            "traffic": None,
+            "flop_model": "42 flops per support bin per evaluation",
+        },
+        "forward_ms": fwd_ms,
+        "forward_backward_ms": fb_ms,
+        "ascent": {
+            "mean_2F_gain": float(gain.mean()),
+            "min_2F_gain": float(gain.min()),
+            "mean_accepted": float(nacc_h.mean()),
+        },
+        "parity": {
+            "candidates": int(idx.size),
+            "max_rel_err_2F": worst,
+            "max_rel_gap_to_scanned_max": scan_gap,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    prob.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +374,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="rect_batched", choices=sorted(WORKLOADS))
     ap.add_argument("--templates", type=int, default=4096)
+    ap.add_argument("--refine-steps", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-btsg", action="store_true")
     ap.add_argument("--plain-templates", type=int, default=256)
@@ -231,6 +395,10 @@ def main():
               "(templates are independent: a multi-GPU run would shard them by cost).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "exp_refine":
+        run_refine(args)
+        return
 
     import torch
 

@@ -174,6 +174,111 @@ int tcw_plan_destroy(tcw_plan *p);
  * 16 in [16, 512], or 0 for the built-in default.  Affects later plans. */
 int tcw_set_rows_per_item(int mode, int rows);
 
+/* ---------------------------------------------------------------------------
+ * Binned detector merge for many templates (lalpulsar
+ * mergeMultiFstatAtomsBinned, called at pyfstat/tcw_fstat_map_funcs.py:706):
+ * template t's bins k = 0 .. N_t-1 cover [t_data[t] + k*TAtom, +TAtom), with
+ * t_data[t] the smallest first timestamp of its segments and
+ * N_t = (largest last timestamp - t_data[t]) / TAtom + 1; each bin holds the
+ * float32 sum of the atoms falling in it, in detector order.
+ *
+ * m_off (host [T+1]) and t_data (host [T]) are always filled; merged bins of
+ * template t go to merged[c*merged_stride + m_off[t] + k].  With merged ==
+ * NULL only the shapes are computed.  atom_bin (device [total atoms], may be
+ * NULL) receives each input atom's absolute merged index m_off[t] + k, which
+ * tcw_gather_bins uses to pull merged-bin gradients back to the atoms.
+ * desc->windows is not read.  The call synchronises `stream`.
+ * ------------------------------------------------------------------------- */
+int tcw_merge_batch(tcw_handle *h, const tcw_batch_desc *desc, const uint32_t *ts,
+                    const float *cols, int64_t col_stride, float *merged,
+                    int64_t merged_stride, int64_t *m_off, uint32_t *t_data,
+                    int64_t *atom_bin, void *stream);
+
+/* out[c*out_stride + j] = in[c*in_stride + atom_bin[j]] for the 7 columns and
+ * j < n_atoms: the adjoint of the merge (each atom's gradient is that of its
+ * bin).  Asynchronous on `stream`. */
+int tcw_gather_bins(const float *in, int64_t in_stride, const int64_t *atom_bin,
+                    int64_t n_atoms, float *out, int64_t out_stride, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Exponential-window candidates on a fixed atom support, with derivatives.
+ *
+ * A candidate is a continuous (t0, tau) of one template together with a fixed
+ * support: merged bins k_lo .. k_hi of that template.  Its window sums are
+ *   A, B, C        = sum_k w_k^2 (a2, b2, ab)_k,
+ *   Fa, Fb         = sum_k w_k   (Fa, Fb)_k,     w_k = exp(-(t_k - t0)/tau),
+ * t_k = t_data + k*TAtom, and F follows from them as in the map kernels
+ * (cond < 1e4 guard, F = 2 otherwise).  At a grid cell with the support the
+ * map uses (bins with t0 <= t_k <= t0 + 3 tau, the window of
+ * pyfstat/pyCUDAkernels/cudaTransientFstatExpWindow.cu:82-102) F equals the
+ * map value.  lnBtSG of the candidate is that of a 1x1 map,
+ * log(70) + F (tcw_fstat_map_funcs.py:196-217 with N_t0 = N_tau = 1).
+ *
+ * On a fixed support every weight carries the common factor exp(t0/tau), and
+ * F is homogeneous of degree 0 in (A, B, C; Fa, Fb) with weights (2; 1), so
+ * dF/dt0 vanishes identically; it is still returned (it is the sum the
+ * backward uses and measures rounding).
+ *
+ * Reference analogue: MCMCTransientSearch evaluates a 1x1 map per likelihood
+ * call (pyfstat/mcmc_based_searches.py:3511-3516); the reference has no
+ * derivatives.
+ * ------------------------------------------------------------------------- */
+#define TCW_REFINE_SAVED 10 /* doubles per candidate in `saved` */
+
+typedef struct {
+    int64_t T;               /* templates */
+    int64_t n_cand;          /* candidates, grouped by template */
+    uint32_t TAtom;
+    uint32_t reserved;
+    const int64_t *m_off;    /* host [T+1]: template t owns merged bins m_off[t] .. m_off[t+1]-1 */
+    const uint32_t *t_data;  /* host [T]: timestamp of bin 0 of each template */
+    const int64_t *cand_off; /* host [T+1]: template t owns candidates cand_off[t] .. cand_off[t+1]-1 */
+    const int32_t *k_lo;     /* host [n_cand]: first support bin, relative to m_off[t] */
+    const int32_t *k_hi;     /* host [n_cand]: last support bin (inclusive) */
+} tcw_refine_desc;
+
+typedef struct tcw_refine_plan tcw_refine_plan;
+
+/* Plan: uploads the supports and a work schedule (host arrays read here
+ * only). */
+int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *desc,
+                           tcw_refine_plan **out);
+int tcw_refine_plan_destroy(tcw_refine_plan *p);
+
+/* Forward: twoF[i] = 2F, lnBtSG[i] = log(70) + F (may be NULL), and
+ * saved[i*TCW_REFINE_SAVED + j] = dF/dt0, dF/dtau, dF/dA, dF/dB, dF/dC,
+ * dF/dFa_re, dF/dFa_im, dF/dFb_re, dF/dFb_im, F (may be NULL).
+ * cols: device merged atoms [TCW_NCOL][col_stride] (tcw_merge_batch layout);
+ * t0, tau: device double [n_cand].  Asynchronous on `stream`. */
+int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride,
+                       const double *t0, const double *tau, double *twoF, double *lnBtSG,
+                       double *saved, void *stream);
+
+/* Backward of tcw_refine_forward for upstream gradients g_twoF, g_lnBtSG
+ * (device double [n_cand], either may be NULL): with g = 2 g_twoF + g_lnBtSG,
+ *   grad_t0[i]  = g_i dF_i/dt0,   grad_tau[i] = g_i dF_i/dtau   (may be NULL),
+ *   grad_cols[c*col_stride + m_off[t] + k] = sum over the template's
+ *       candidates i whose support holds k of g_i dF_i/dS_c w_ik^{p_c}
+ *   (p = 2 for a2, b2, ab; 1 for Fa, Fb), float32; every bin of every
+ *   template is written (zero outside the supports); may be NULL.
+ * saved is the forward's.  Asynchronous on `stream`. */
+int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau,
+                        const double *saved, const double *g_twoF, const double *g_lnBtSG,
+                        float *grad_cols, int64_t col_stride, double *grad_t0,
+                        double *grad_tau, void *stream);
+
+/* n_steps of gradient ascent of F in log(tau) per candidate, on the fixed
+ * support, in one launch: a step proposes s' = s + eta dF/ds (s = log tau,
+ * clamped to [log tau_lo, log tau_hi]; tau_lo/tau_hi may be NULL for no
+ * bound), is accepted if F rises (eta *= 1.5) and rejected otherwise
+ * (eta *= 0.5); the first step moves s by step0.  t0 is not moved (dF/dt0 = 0
+ * on a fixed support).  tau is updated in place; twoF receives the final 2F
+ * and n_accept (may be NULL) the accepted steps.  Asynchronous on `stream`. */
+int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
+                      const double *t0, double *tau, const double *tau_lo,
+                      const double *tau_hi, int n_steps, double step0, double *twoF,
+                      int32_t *n_accept, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

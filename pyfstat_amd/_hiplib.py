@@ -43,7 +43,16 @@ EXPORTED_SYMBOLS = (
     "tcw_batch_run_timed",
     "tcw_plan_destroy",
     "tcw_set_rows_per_item",
+    "tcw_merge_batch",
+    "tcw_gather_bins",
+    "tcw_refine_plan_create",
+    "tcw_refine_plan_destroy",
+    "tcw_refine_forward",
+    "tcw_refine_backward",
+    "tcw_refine_ascent",
 )
+
+TCW_REFINE_SAVED = 10
 
 
 class Window(ctypes.Structure):
@@ -96,6 +105,20 @@ class BatchDesc(ctypes.Structure):
         ("seg_tfirst", ctypes.c_void_p),
         ("seg_tlast", ctypes.c_void_p),
         ("windows", ctypes.c_void_p),
+    ]
+
+
+class RefineDesc(ctypes.Structure):
+    _fields_ = [
+        ("T", ctypes.c_int64),
+        ("n_cand", ctypes.c_int64),
+        ("TAtom", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("m_off", ctypes.c_void_p),
+        ("t_data", ctypes.c_void_p),
+        ("cand_off", ctypes.c_void_p),
+        ("k_lo", ctypes.c_void_p),
+        ("k_hi", ctypes.c_void_p),
     ]
 
 
@@ -157,6 +180,22 @@ def load():
     lib.tcw_plan_destroy.argtypes = [vp]
     lib.tcw_set_rows_per_item.restype = ctypes.c_int
     lib.tcw_set_rows_per_item.argtypes = [ctypes.c_int, ctypes.c_int]
+    i64 = ctypes.c_int64
+    lib.tcw_merge_batch.restype = ctypes.c_int
+    lib.tcw_merge_batch.argtypes = [vp, P(BatchDesc), vp, vp, i64, vp, i64, vp, vp, vp, vp]
+    lib.tcw_gather_bins.restype = ctypes.c_int
+    lib.tcw_gather_bins.argtypes = [vp, i64, vp, i64, vp, i64, vp]
+    lib.tcw_refine_plan_create.restype = ctypes.c_int
+    lib.tcw_refine_plan_create.argtypes = [vp, P(RefineDesc), P(vp)]
+    lib.tcw_refine_plan_destroy.restype = ctypes.c_int
+    lib.tcw_refine_plan_destroy.argtypes = [vp]
+    lib.tcw_refine_forward.restype = ctypes.c_int
+    lib.tcw_refine_forward.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, vp]
+    lib.tcw_refine_backward.restype = ctypes.c_int
+    lib.tcw_refine_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp]
+    lib.tcw_refine_ascent.restype = ctypes.c_int
+    lib.tcw_refine_ascent.argtypes = [vp, vp, i64, vp, vp, vp, vp, ctypes.c_int, ctypes.c_double,
+                                      vp, vp, vp]
     _lib = lib
     return lib
 

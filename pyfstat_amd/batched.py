@@ -71,6 +71,7 @@ class BatchedTransientMaps:
             self.flags |= _hiplib.TCW_FLAG_GENERIC
         self.BtSG = bool(BtSG)
         self.lal_strict = bool(lal_strict)
+        self.TAtom = int(TAtom)
         desc = _hiplib.BatchDesc()
         desc.T = self.T
         desc.TAtom = int(TAtom)

@@ -12,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "tcw_map.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("tcw_map.hip", "tcw_refine.hip")]
 INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(HERE, "_lib", "libtcw_hip.so")
 
@@ -37,10 +37,10 @@ def hipcc():
 
 
 def build(verbose=False):
-    """Compile csrc/tcw_map.hip into _lib/libtcw_hip.so (always recompiles)."""
+    """Compile csrc/*.hip into _lib/libtcw_hip.so (always recompiles)."""
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", "-o", tmp, SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", "-o", tmp, *SRCS]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

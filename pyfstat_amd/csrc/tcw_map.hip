@@ -28,6 +28,7 @@
 #include <string>
 #include <vector>
 
+#include "tcw_common.h"
 #include "tcw_hip.h"
 
 namespace {
@@ -1287,6 +1288,83 @@ __global__ __launch_bounds__(WG) void finalize_kernel(
     }
 }
 
+// ----------------------------------------------------------------------------
+// binned merge for many templates (tcw_merge_batch) and its adjoint
+// ----------------------------------------------------------------------------
+
+struct MergeTmpl {
+    int64_t seg_begin, seg_end;  // into the SegDev table
+    int64_t out_off;             // first merged bin of the template
+    uint32_t N, t_min;
+};
+
+// One thread per merged bin: per detector segment a binary search for the
+// bin's first atom, then float32 sums over the bin's atoms, detectors in
+// order -- the accumulation order of mergeMultiFstatAtomsBinned, so the
+// result is bit-identical to the stage kernel's merge.
+__global__ __launch_bounds__(256) void merge_kernel(
+    const MergeTmpl *__restrict__ mt, const SegDev *__restrict__ segs, int64_t T,
+    const uint32_t *__restrict__ ts, const float *__restrict__ cols, int64_t cstride,
+    uint32_t TA, float *__restrict__ out, int64_t ostride) {
+    for (int64_t t = blockIdx.y; t < T; t += gridDim.y) {
+        const MergeTmpl m = mt[t];
+        const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+        if (b >= m.N) continue;
+        float v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = 0.0f;
+        const uint64_t t_lo = (uint64_t)m.t_min + (uint64_t)b * TA, t_hi = t_lo + TA;
+        for (int64_t s = m.seg_begin; s < m.seg_end; s++) {
+            const SegDev sg = segs[s];
+            int64_t lo = sg.begin, hi = sg.end;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((uint64_t)ts[mid] < t_lo)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            for (int64_t k = lo; k < sg.end && (uint64_t)ts[k] < t_hi; k++) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) v[c] += cols[c * cstride + k];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) out[c * ostride + m.out_off + b] = v[c];
+    }
+}
+
+// atom_bin[k] = out_off + (ts[k] - t_min) / TA for every atom of every segment;
+// segment s of template t is blockIdx.y-strided, atoms over x.
+struct MergeSeg {
+    int64_t begin, end, out_off;
+    uint32_t t_min, pad;
+};
+
+__global__ __launch_bounds__(256) void atom_bin_kernel(const MergeSeg *__restrict__ ms,
+                                                       int64_t S, const uint32_t *__restrict__ ts,
+                                                       uint32_t TA, int64_t *__restrict__ atom_bin) {
+    for (int64_t s = blockIdx.y; s < S; s += gridDim.y) {
+        const MergeSeg g = ms[s];
+        for (int64_t k = g.begin + (int64_t)blockIdx.x * 256 + threadIdx.x; k < g.end;
+             k += (int64_t)gridDim.x * 256)
+            atom_bin[k] = g.out_off + (int64_t)((ts[k] - g.t_min) / TA);
+    }
+}
+
+__global__ __launch_bounds__(256) void gather_bins_kernel(const float *__restrict__ in,
+                                                          int64_t istride,
+                                                          const int64_t *__restrict__ atom_bin,
+                                                          int64_t n, float *__restrict__ out,
+                                                          int64_t ostride) {
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n;
+         j += (int64_t)gridDim.x * 256) {
+        const int64_t b = atom_bin[j];
+#pragma unroll
+        for (int c = 0; c < NC; c++) out[c * ostride + j] = in[c * istride + b];
+    }
+}
+
 __global__ void zero_ints_kernel(int *p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -1296,44 +1374,9 @@ __global__ void zero_ints_kernel(int *p, int64_t n) {
 // host side
 // ----------------------------------------------------------------------------
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-    g_err = msg;
-    return code;
-}
-
-#define TCW_HIP_TRY(expr)                                                             \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            return fail(TCW_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    int ensure(size_t bytes) {
-        if (bytes <= cap) return TCW_OK;
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max<size_t>(bytes, 256);
-        if (hipMalloc(&p, want) != hipSuccess) {
-            p = nullptr;
-            return fail(TCW_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
-        }
-        cap = want;
-        return TCW_OK;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
+using tcwc::DevBuf;
+using tcwc::fail;
+using tcwc::set_device;
 
 int g_rb_override[MODE_COUNT] = {0, 0, 0, 0};
 
@@ -1398,11 +1441,17 @@ struct tcw_handle {
     tcw_plan single;
     tcw_stats *d_stats = nullptr;
     tcw_stats *h_stats = nullptr;  // pinned
+    DevBuf d_mtab, d_mseg, d_msegs;  // tcw_merge_batch tables
     ~tcw_handle() {
         if (d_stats) (void)hipFree(d_stats);
         if (h_stats) (void)hipHostFree(h_stats);
+        d_mtab.release();
+        d_mseg.release();
+        d_msegs.release();
     }
 };
+
+int tcwc::handle_device(const tcw_handle *h) { return h->device; }
 
 namespace {
 
@@ -1705,20 +1754,13 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     return TCW_OK;
 }
 
-int set_device(int device) {
-    int cur = -1;
-    TCW_HIP_TRY(hipGetDevice(&cur));
-    if (cur != device) TCW_HIP_TRY(hipSetDevice(device));
-    return TCW_OK;
-}
-
 }  // namespace
 
 extern "C" {
 
 const char *tcw_version(void) { return "tcw_hip 0.1 (gfx950)"; }
 
-const char *tcw_last_error(void) { return g_err.c_str(); }
+const char *tcw_last_error(void) { return tcwc::g_err.c_str(); }
 
 int tcw_device_count(void) {
     int n = 0;
@@ -1923,6 +1965,99 @@ int tcw_set_rows_per_item(int mode, int rows) {
         return fail(TCW_ERR_INVALID, "rows per item must be a multiple of 16 in [16, " +
                                          std::to_string(maxr) + "]");
     g_rb_override[mode] = rows;
+    return TCW_OK;
+}
+
+int tcw_merge_batch(tcw_handle *h, const tcw_batch_desc *d, const uint32_t *ts,
+                    const float *cols, int64_t col_stride, float *merged, int64_t merged_stride,
+                    int64_t *m_off, uint32_t *t_data, int64_t *atom_bin, void *stream) {
+    if (!h || !d || !m_off || !t_data) return fail(TCW_ERR_INVALID, "tcw_merge_batch: null argument");
+    if (d->T <= 0) return fail(TCW_ERR_INVALID, "no templates");
+    if (d->TAtom < 2) return fail(TCW_ERR_INVALID, "TAtom must be at least 2 seconds.");
+    std::vector<MergeTmpl> tab((size_t)d->T);
+    std::vector<SegDev> segs;
+    std::vector<MergeSeg> mseg;
+    int64_t off = 0, maxN = 0;
+    for (int64_t t = 0; t < d->T; t++) {
+        const int64_t s0 = d->seg_begin[t], s1 = d->seg_begin[t + 1];
+        if (s1 <= s0 || s1 - s0 > TCW_MAX_DETECTORS)
+            return fail(TCW_ERR_INVALID, "template " + std::to_string(t) + ": needs 1.." +
+                                             std::to_string(TCW_MAX_DETECTORS) +
+                                             " detector segments");
+        uint32_t tmin = 0xffffffffu, tmax = 0;
+        for (int64_t s = s0; s < s1; s++) {
+            if (d->seg_off[s + 1] <= d->seg_off[s])
+                return fail(TCW_ERR_INVALID, "template " + std::to_string(t) + ": empty detector");
+            if (d->seg_tlast[s] < d->seg_tfirst[s])
+                return fail(TCW_ERR_INVALID, "template " + std::to_string(t) +
+                                                 ": timestamps not in ascending order");
+            tmin = std::min(tmin, d->seg_tfirst[s]);
+            tmax = std::max(tmax, d->seg_tlast[s]);
+        }
+        const int64_t N = (int64_t)((tmax - tmin) / d->TAtom) + 1;
+        tab[(size_t)t] = MergeTmpl{(int64_t)segs.size(), (int64_t)segs.size() + (s1 - s0), off,
+                                   (uint32_t)N, tmin};
+        for (int64_t s = s0; s < s1; s++) {
+            segs.push_back(SegDev{d->seg_off[s], d->seg_off[s + 1]});
+            mseg.push_back(MergeSeg{d->seg_off[s], d->seg_off[s + 1], off, tmin, 0});
+        }
+        m_off[t] = off;
+        t_data[t] = tmin;
+        off += N;
+        maxN = std::max(maxN, N);
+    }
+    m_off[d->T] = off;
+    if (!merged) return TCW_OK;
+    if (!ts || !cols) return fail(TCW_ERR_INVALID, "tcw_merge_batch: null atoms");
+    if (merged_stride < off)
+        return fail(TCW_ERR_INVALID, "merged_stride " + std::to_string(merged_stride) +
+                                         " < merged bins " + std::to_string(off));
+    int64_t max_end = 0;
+    for (const auto &g : segs) max_end = std::max(max_end, g.end);
+    if (col_stride < max_end) return fail(TCW_ERR_INVALID, "col_stride smaller than the atom count");
+    int rc = set_device(h->device);
+    if (rc) return rc;
+    if ((rc = h->d_mtab.ensure(tab.size() * sizeof(MergeTmpl))) ||
+        (rc = h->d_msegs.ensure(segs.size() * sizeof(SegDev))) ||
+        (rc = h->d_mseg.ensure(mseg.size() * sizeof(MergeSeg))))
+        return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // the previous call's kernels may still read the tables
+    TCW_HIP_TRY(hipStreamSynchronize(st));
+    TCW_HIP_TRY(hipMemcpy(h->d_mtab.p, tab.data(), tab.size() * sizeof(MergeTmpl),
+                          hipMemcpyHostToDevice));
+    TCW_HIP_TRY(hipMemcpy(h->d_msegs.p, segs.data(), segs.size() * sizeof(SegDev),
+                          hipMemcpyHostToDevice));
+    TCW_HIP_TRY(hipMemcpy(h->d_mseg.p, mseg.data(), mseg.size() * sizeof(MergeSeg),
+                          hipMemcpyHostToDevice));
+    const unsigned gy = (unsigned)std::min<int64_t>(d->T, 65535);
+    hipLaunchKernelGGL(merge_kernel, dim3((unsigned)((maxN + 255) / 256), gy), dim3(256), 0, st,
+                       (const MergeTmpl *)h->d_mtab.p, (const SegDev *)h->d_msegs.p, d->T, ts,
+                       cols, col_stride, d->TAtom, merged, merged_stride);
+    if (atom_bin) {
+        int64_t maxS = 0;
+        for (const auto &g : mseg) maxS = std::max(maxS, g.end - g.begin);
+        const unsigned sy = (unsigned)std::min<size_t>(mseg.size(), 65535);
+        const unsigned sx = (unsigned)std::min<int64_t>((maxS + 255) / 256, 1024);
+        hipLaunchKernelGGL(atom_bin_kernel, dim3(sx, sy), dim3(256), 0, st,
+                           (const MergeSeg *)h->d_mseg.p, (int64_t)mseg.size(), ts, d->TAtom,
+                           atom_bin);
+    }
+    TCW_HIP_TRY(hipGetLastError());
+    TCW_HIP_TRY(hipStreamSynchronize(st));
+    return TCW_OK;
+}
+
+int tcw_gather_bins(const float *in, int64_t in_stride, const int64_t *atom_bin, int64_t n_atoms,
+                    float *out, int64_t out_stride, void *stream) {
+    if (n_atoms < 0 || out_stride < n_atoms)
+        return fail(TCW_ERR_INVALID, "tcw_gather_bins: bad sizes");
+    if (n_atoms == 0) return TCW_OK;
+    if (!in || !atom_bin || !out) return fail(TCW_ERR_INVALID, "tcw_gather_bins: null argument");
+    const unsigned g = (unsigned)std::min<int64_t>((n_atoms + 255) / 256, 65536);
+    hipLaunchKernelGGL(gather_bins_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, in,
+                       in_stride, atom_bin, n_atoms, out, out_stride);
+    TCW_HIP_TRY(hipGetLastError());
     return TCW_OK;
 }
 

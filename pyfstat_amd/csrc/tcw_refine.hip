@@ -1,0 +1,509 @@
+// tcw_refine.hip -- exponential-window candidates on a fixed atom support
+// (gfx950): 2F and lnBtSG with analytic first derivatives, the backward pass
+// to the merged atoms, and a multi-step gradient ascent in tau, in one launch.
+//
+// Window sums of a candidate (t0, tau) with support bins k_lo..k_hi:
+//   S_c = sum_k w_k^{p_c} x_{c,k},  w_k = exp(-(t_k - t0)/tau),  p = (2,2,2,1,1,1,1)
+// for x = (a2, b2, ab, Fa_re, Fa_im, Fb_re, Fb_im), the weighting of
+// pyfstat/pyCUDAkernels/cudaTransientFstatExpWindow.cu:82-102 (win for Fa/Fb,
+// win^2 for A/B/C), and F from them as at :109-129 (cond < 1e4 guard, F = 2
+// otherwise).  Derivatives, with D = AB - C^2 and F = (B|Fa|^2 + A|Fb|^2 -
+// 2C Re(Fa Fb*))/D:
+//   dF/dA = (|Fb|^2 - F B)/D, dF/dB = (|Fa|^2 - F A)/D, dF/dC = 2(F C - Re(Fa Fb*))/D,
+//   dF/dFa = 2(B Fa - C Fb)/D, dF/dFb = 2(A Fb - C Fa)/D   (re and im parts),
+//   dS_c/dt0 = p_c S_c / tau,  dS_c/dtau = p_c U_c / tau^2,  U_c = sum_k w_k^{p_c} (t_k - t0) x_{c,k}.
+// All sums and the 2x2 algebra run in FP64 (on CDNA4 the FP64 VALU rate equals
+// the unpacked FP32 rate); weights come from one exp per lane and a geometric
+// recurrence along the support (t_k is a regular grid).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tcw_common.h"
+#include "tcw_hip.h"
+
+using tcwc::DevBuf;
+using tcwc::fail;
+using tcwc::set_device;
+
+namespace {
+
+constexpr int NC = TCW_NCOL;
+constexpr int RWG = 256;             // 4 waves
+constexpr int CAND_PER_ITEM = 8;     // candidates per work item (2 per wave)
+constexpr int LDS_ATOMS_CAP = 2176;  // staged support bins: 7*4*2176 = 60928 B
+constexpr int BWD_BINS = 256;        // bins per backward block
+constexpr double COND_MAX = 1e4;
+constexpr double QCOND = (COND_MAX + 1.0) * (COND_MAX + 1.0) / COND_MAX;  // cond < K <=> T^2 < Q D
+constexpr double LN70 = 4.248495242049359;  // log(70)
+constexpr int NSAVED = TCW_REFINE_SAVED;
+
+struct RTmpl {
+    int64_t m_off;             // first merged bin
+    int64_t cand_beg, cand_end;
+    uint32_t t_data, N;
+};
+
+struct RItem {
+    int64_t t;      // template
+    int64_t cbeg;   // first candidate (caller order, grouped by template)
+    int32_t ncand;  // candidates in the item
+    int32_t u_lo;   // first staged bin (relative to the template)
+    int32_t u_len;  // staged bins
+    int32_t in_lds; // 1: support union staged in LDS, 0: read from global memory
+};
+
+struct BItem {
+    int64_t t;
+    int32_t b0, nb;  // bins b0 .. b0+nb-1 of template t
+};
+
+static_assert(sizeof(RItem) == 32, "RItem layout");
+
+struct Eval {
+    double F, dFdt0, dFdtau;
+    double g[NC];
+};
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum 16 per-lane values over the wave by recursive halving: each round a lane
+// hands the half of its values it does not keep to its partner, so 8+4+2+1
+// exchanges replace 16*6.  Afterwards lane L holds the total of value (L>>2)&15.
+__device__ __forceinline__ double wave_transpose_sum16(double (&v)[16], int lane) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int off = 32 >> r;
+        const int h = 8 >> r;
+        const bool up = (lane & off) != 0;
+#pragma unroll
+        for (int i = 0; i < h; i++) {
+            const double send = up ? v[i] : v[i + h];
+            const double keep = up ? v[i + h] : v[i];
+            v[i] = keep + __shfl_xor(send, off, 64);
+        }
+    }
+    double x = v[0];
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    return x;
+}
+
+// F and its derivatives for one candidate, evaluated by one wave.  Atom (c, k)
+// of the template is src[c*sstride + k - sbase].  Every lane returns the same
+// values (wave-uniform control flow for the callers).
+__device__ Eval eval_candidate(const float *src, int64_t sstride, int sbase, int kb, int ke,
+                               double t_data, double TA, double t0, double tau, int lane) {
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = 0.0;
+    const double inv_tau = 1.0 / tau;
+    const int k0 = kb + lane;
+    double u = fma((double)k0, TA, t_data - t0);  // t_k - t0
+    double w = exp(-u * inv_tau);
+    const double r64 = exp(-64.0 * TA * inv_tau);
+    const double du = 64.0 * TA;
+    for (int k = k0; k <= ke; k += 64) {
+        const float *a = src + (k - sbase);
+        const double w2 = w * w;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double x = (c < 3 ? w2 : w) * (double)a[c * sstride];
+            v[c] += x;
+            v[NC + c] = fma(u, x, v[NC + c]);
+        }
+        w *= r64;
+        u += du;
+    }
+    const double red = wave_transpose_sum16(v, lane);
+    double s[2 * NC];
+#pragma unroll
+    for (int j = 0; j < 2 * NC; j++) s[j] = readlane_d(red, 4 * j);
+
+    Eval e;
+    const double A = s[0], B = s[1], C = s[2];
+    const double far = s[3], fai = s[4], fbr = s[5], fbi = s[6];
+    const double D = fma(A, B, -(C * C));
+    const double T = A + B;
+    if (!((T > 0.0) && (T * T < QCOND * D))) {
+        e.F = 2.0;
+        e.dFdt0 = e.dFdtau = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) e.g[c] = 0.0;
+        return e;
+    }
+    const double iD = 1.0 / D;
+    const double p = fma(far, far, fai * fai);
+    const double q = fma(fbr, fbr, fbi * fbi);
+    const double rr = fma(far, fbr, fai * fbi);
+    const double F = fma(B, p, fma(A, q, (-2.0 * C) * rr)) * iD;
+    e.F = F;
+    e.g[0] = (q - F * B) * iD;
+    e.g[1] = (p - F * A) * iD;
+    e.g[2] = 2.0 * (F * C - rr) * iD;
+    e.g[3] = 2.0 * (B * far - C * fbr) * iD;
+    e.g[4] = 2.0 * (B * fai - C * fbi) * iD;
+    e.g[5] = 2.0 * (A * fbr - C * far) * iD;
+    e.g[6] = 2.0 * (A * fbi - C * fai) * iD;
+    double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const double pc = c < 3 ? 2.0 : 1.0;
+        d0 = fma(pc * s[c], e.g[c], d0);
+        d1 = fma(pc * s[NC + c], e.g[c], d1);
+    }
+    e.dFdt0 = d0 * inv_tau;
+    e.dFdtau = d1 * inv_tau * inv_tau;
+    return e;
+}
+
+// Stage the item's support union (component-major) into LDS when it fits and
+// return where atom (c, k) lives.
+__device__ __forceinline__ const float *stage_support(const RItem &it, const RTmpl &tp,
+                                                      const float *__restrict__ cols,
+                                                      int64_t cstride, float *lds,
+                                                      int64_t &sstride, int &sbase) {
+    if (it.in_lds) {
+        const float *g = cols + tp.m_off + it.u_lo;
+        for (int c = 0; c < NC; c++)
+            for (int k = threadIdx.x; k < it.u_len; k += RWG) lds[c * it.u_len + k] = g[c * cstride + k];
+        __syncthreads();
+        sstride = it.u_len;
+        sbase = it.u_lo;
+        return lds;
+    }
+    sstride = cstride;
+    sbase = 0;
+    return cols + tp.m_off;
+}
+
+__global__ __launch_bounds__(RWG) void refine_forward_kernel(
+    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, const double *__restrict__ tauv, double *__restrict__ twoF,
+    double *__restrict__ lnB, double *__restrict__ saved) {
+    extern __shared__ float lds[];
+    const RItem it = items[blockIdx.x];
+    const RTmpl tp = tm[it.t];
+    int64_t sstride;
+    int sbase;
+    const float *src = stage_support(it, tp, cols, cstride, lds, sstride, sbase);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = wave; j < it.ncand; j += RWG / 64) {
+        const int64_t ci = it.cbeg + j;
+        const Eval e = eval_candidate(src, sstride, sbase, k_lo[ci], k_hi[ci], (double)tp.t_data,
+                                      TA, t0v[ci], tauv[ci], lane);
+        if (lane == 0) {
+            twoF[ci] = 2.0 * e.F;
+            if (lnB) lnB[ci] = LN70 + e.F;
+        }
+        if (saved && lane < NSAVED) {
+            double x = e.F;
+            if (lane == 0) x = e.dFdt0;
+            if (lane == 1) x = e.dFdtau;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                if (lane == 2 + c) x = e.g[c];
+            saved[ci * NSAVED + lane] = x;
+        }
+    }
+}
+
+__global__ __launch_bounds__(RWG) void refine_ascent_kernel(
+    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
+    const double *__restrict__ tau_hi, int n_steps, double step0, double *__restrict__ twoF,
+    int32_t *__restrict__ n_accept) {
+    extern __shared__ float lds[];
+    const RItem it = items[blockIdx.x];
+    const RTmpl tp = tm[it.t];
+    int64_t sstride;
+    int sbase;
+    const float *src = stage_support(it, tp, cols, cstride, lds, sstride, sbase);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = wave; j < it.ncand; j += RWG / 64) {
+        const int64_t ci = it.cbeg + j;
+        const int kb = k_lo[ci], ke = k_hi[ci];
+        const double t0 = t0v[ci];
+        double tau = tauv[ci];
+        int acc = 0;
+        double F2 = NAN;
+        if (tau > 0.0) {
+            const double lo = tau_lo ? log(tau_lo[ci]) : -INFINITY;
+            const double hi = tau_hi ? log(tau_hi[ci]) : INFINITY;
+            Eval e = eval_candidate(src, sstride, sbase, kb, ke, (double)tp.t_data, TA, t0, tau,
+                                    lane);
+            double s = log(tau);
+            double g = tau * e.dFdtau;  // dF/dlog(tau)
+            double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
+            for (int step = 0; step < n_steps; step++) {
+                const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
+                if (!(sn != s)) {  // no move possible with this step size
+                    eta *= 0.5;
+                    continue;
+                }
+                const double taun = exp(sn);
+                const Eval en = eval_candidate(src, sstride, sbase, kb, ke, (double)tp.t_data, TA,
+                                               t0, taun, lane);
+                if (en.F > e.F) {
+                    s = sn;
+                    tau = taun;
+                    e = en;
+                    g = tau * e.dFdtau;
+                    eta *= 1.5;
+                    acc++;
+                } else {
+                    eta *= 0.5;
+                }
+            }
+            F2 = 2.0 * e.F;
+        }
+        if (lane == 0) {
+            tauv[ci] = tau;
+            twoF[ci] = F2;
+            if (n_accept) n_accept[ci] = acc;
+        }
+    }
+}
+
+// grad_cols for bins b0 .. b0+nb-1 of one template: sum over the template's
+// candidates of g_i dF_i/dS_c w_ik^{p_c}.  Candidate data are uniform loads;
+// weights in FP32 (exp of an FP64 argument rounded once).
+__global__ __launch_bounds__(BWD_BINS) void refine_backward_kernel(
+    const RTmpl *__restrict__ tm, const BItem *__restrict__ bitems, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, double TA, const double *__restrict__ t0v,
+    const double *__restrict__ tauv, const double *__restrict__ saved,
+    const double *__restrict__ g2F, const double *__restrict__ glnB, float *__restrict__ grad,
+    int64_t cstride) {
+    const BItem bi = bitems[blockIdx.x];
+    const RTmpl tp = tm[bi.t];
+    const int k = bi.b0 + (int)threadIdx.x;
+    float acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = 0.0f;
+    const int blo = bi.b0, bhi = bi.b0 + bi.nb - 1;
+    for (int64_t i = tp.cand_beg; i < tp.cand_end; i++) {
+        const int kb = k_lo[i], ke = k_hi[i];
+        if (ke < blo || kb > bhi) continue;  // uniform
+        const double gF = (g2F ? 2.0 * g2F[i] : 0.0) + (glnB ? glnB[i] : 0.0);
+        if (gF == 0.0) continue;
+        const double tau = tauv[i];
+        const double x0 = (fma((double)blo, TA, (double)tp.t_data) - t0v[i]) / tau;
+        const double dx = TA / tau;
+        if (k >= kb && k <= ke) {
+            const float w = expf((float)-fma((double)threadIdx.x, dx, x0));
+            const float w2 = w * w;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const float coef = (float)(gF * saved[i * NSAVED + 2 + c]);
+                acc[c] = fmaf(coef, c < 3 ? w2 : w, acc[c]);
+            }
+        }
+    }
+    if ((int)threadIdx.x < bi.nb) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) grad[c * cstride + tp.m_off + k] = acc[c];
+    }
+}
+
+__global__ __launch_bounds__(256) void refine_param_grad_kernel(
+    int64_t n, const double *__restrict__ saved, const double *__restrict__ g2F,
+    const double *__restrict__ glnB, double *__restrict__ gt0, double *__restrict__ gtau) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double gF = (g2F ? 2.0 * g2F[i] : 0.0) + (glnB ? glnB[i] : 0.0);
+    if (gt0) gt0[i] = gF * saved[i * NSAVED + 0];
+    if (gtau) gtau[i] = gF * saved[i * NSAVED + 1];
+}
+
+}  // namespace
+
+struct tcw_refine_plan {
+    int device = 0;
+    int64_t T = 0, n_cand = 0, total_bins = 0;
+    uint32_t TAtom = 0;
+    int lds_bytes = 0;
+    int64_t n_items = 0, n_bitems = 0;
+    DevBuf d_tmpl, d_items, d_bitems, d_klo, d_khi;
+    ~tcw_refine_plan() {
+        d_tmpl.release();
+        d_items.release();
+        d_bitems.release();
+        d_klo.release();
+        d_khi.release();
+    }
+};
+
+namespace {
+
+int check_plan(const tcw_refine_plan *p) {
+    if (!p) return fail(TCW_ERR_INVALID, "null refine plan");
+    return set_device(p->device);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_plan **out) {
+    if (!h || !d || !out) return fail(TCW_ERR_INVALID, "tcw_refine_plan_create: null argument");
+    if (d->T <= 0) return fail(TCW_ERR_INVALID, "no templates");
+    if (d->n_cand < 0 || d->n_cand > INT32_MAX)
+        return fail(TCW_ERR_INVALID, "candidate count out of range");
+    if (d->TAtom < 2) return fail(TCW_ERR_INVALID, "TAtom must be at least 2 seconds.");
+    if (!d->m_off || !d->t_data || !d->cand_off || (d->n_cand > 0 && (!d->k_lo || !d->k_hi)))
+        return fail(TCW_ERR_INVALID, "tcw_refine_plan_create: null array");
+    if (d->m_off[0] < 0 || d->cand_off[0] != 0 || d->cand_off[d->T] != d->n_cand)
+        return fail(TCW_ERR_INVALID, "cand_off must run from 0 to n_cand");
+    std::vector<RTmpl> tm((size_t)d->T);
+    std::vector<RItem> items;
+    std::vector<BItem> bitems;
+    int max_len = 0;
+    for (int64_t t = 0; t < d->T; t++) {
+        const int64_t N = d->m_off[t + 1] - d->m_off[t];
+        if (N <= 0 || N > INT32_MAX / 2)
+            return fail(TCW_ERR_INVALID, "template " + std::to_string(t) + ": bad bin range");
+        const int64_t c0 = d->cand_off[t], c1 = d->cand_off[t + 1];
+        if (c1 < c0) return fail(TCW_ERR_INVALID, "cand_off not ascending");
+        tm[(size_t)t] = RTmpl{d->m_off[t], c0, c1, d->t_data[t], (uint32_t)N};
+        const double t_last = (double)d->t_data[t] + (double)(N - 1) * d->TAtom;
+        if (t_last >= 4294967296.0)
+            return fail(TCW_ERR_INVALID, "template " + std::to_string(t) + ": bins pass 2^32 s");
+        for (int64_t i = c0; i < c1; i++)
+            if (d->k_lo[i] < 0 || d->k_hi[i] < d->k_lo[i] || d->k_hi[i] >= N)
+                return fail(TCW_ERR_INVALID, "candidate " + std::to_string(i) +
+                                                 ": support outside its template's bins");
+        // items: consecutive candidates while the union fits in LDS
+        int64_t i = c0;
+        while (i < c1) {
+            int lo = d->k_lo[i], hi = d->k_hi[i];
+            int64_t j = i + 1;
+            while (j < c1 && j - i < CAND_PER_ITEM) {
+                const int nlo = std::min(lo, d->k_lo[j]), nhi = std::max(hi, d->k_hi[j]);
+                if (nhi - nlo + 1 > LDS_ATOMS_CAP && hi - lo + 1 <= LDS_ATOMS_CAP) break;
+                lo = nlo;
+                hi = nhi;
+                j++;
+            }
+            const int len = hi - lo + 1;
+            const int in_lds = len <= LDS_ATOMS_CAP ? 1 : 0;
+            items.push_back(RItem{t, i, (int32_t)(j - i), lo, len, in_lds});
+            if (in_lds) max_len = std::max(max_len, len);
+            i = j;
+        }
+        // every bin gets a backward slot (templates without candidates get zeros)
+        for (int64_t b = 0; b < N; b += BWD_BINS)
+            bitems.push_back(BItem{t, (int32_t)b, (int32_t)std::min<int64_t>(BWD_BINS, N - b)});
+    }
+    const int device = tcwc::handle_device(h);
+    int rc = set_device(device);
+    if (rc) return rc;
+    auto *p = new tcw_refine_plan;
+    p->device = device;
+    p->T = d->T;
+    p->n_cand = d->n_cand;
+    p->TAtom = d->TAtom;
+    p->total_bins = d->m_off[d->T];
+    p->lds_bytes = (int)(max_len * NC * sizeof(float));
+    p->n_items = (int64_t)items.size();
+    p->n_bitems = (int64_t)bitems.size();
+    const size_t nc = (size_t)std::max<int64_t>(d->n_cand, 1);
+    if ((rc = p->d_tmpl.ensure(tm.size() * sizeof(RTmpl))) ||
+        (rc = p->d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(RItem))) ||
+        (rc = p->d_bitems.ensure(bitems.size() * sizeof(BItem))) ||
+        (rc = p->d_klo.ensure(nc * sizeof(int32_t))) || (rc = p->d_khi.ensure(nc * sizeof(int32_t)))) {
+        delete p;
+        return rc;
+    }
+    hipError_t e = hipMemcpy(p->d_tmpl.p, tm.data(), tm.size() * sizeof(RTmpl), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpy(p->d_items.p, items.data(), items.size() * sizeof(RItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_bitems.p, bitems.data(), bitems.size() * sizeof(BItem),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess && d->n_cand > 0)
+        e = hipMemcpy(p->d_klo.p, d->k_lo, (size_t)d->n_cand * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && d->n_cand > 0)
+        e = hipMemcpy(p->d_khi.p, d->k_hi, (size_t)d->n_cand * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(TCW_ERR_DEVICE, std::string("refine plan upload: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    return TCW_OK;
+}
+
+int tcw_refine_plan_destroy(tcw_refine_plan *p) {
+    if (p) (void)set_device(p->device);
+    delete p;
+    return TCW_OK;
+}
+
+int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride, const double *t0,
+                       const double *tau, double *twoF, double *lnBtSG, double *saved,
+                       void *stream) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (p->n_cand == 0) return TCW_OK;
+    if (!cols || !t0 || !tau || !twoF) return fail(TCW_ERR_INVALID, "tcw_refine_forward: null argument");
+    if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
+    hipLaunchKernelGGL(refine_forward_kernel, dim3((unsigned)p->n_items), dim3(RWG), p->lds_bytes,
+                       (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p, (const RItem *)p->d_items.p,
+                       (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols, col_stride,
+                       (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
+    TCW_HIP_TRY(hipGetLastError());
+    return TCW_OK;
+}
+
+int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau, const double *saved,
+                        const double *g_twoF, const double *g_lnBtSG, float *grad_cols,
+                        int64_t col_stride, double *grad_t0, double *grad_tau, void *stream) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!t0 || !tau || !saved) return fail(TCW_ERR_INVALID, "tcw_refine_backward: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (grad_cols) {
+        if (col_stride < p->total_bins)
+            return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
+        hipLaunchKernelGGL(refine_backward_kernel, dim3((unsigned)p->n_bitems), dim3(BWD_BINS), 0, st,
+                           (const RTmpl *)p->d_tmpl.p, (const BItem *)p->d_bitems.p,
+                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p,
+                           (double)p->TAtom, t0, tau, saved, g_twoF, g_lnBtSG, grad_cols, col_stride);
+    }
+    if ((grad_t0 || grad_tau) && p->n_cand > 0)
+        hipLaunchKernelGGL(refine_param_grad_kernel, dim3((unsigned)((p->n_cand + 255) / 256)),
+                           dim3(256), 0, st, p->n_cand, saved, g_twoF, g_lnBtSG, grad_t0, grad_tau);
+    TCW_HIP_TRY(hipGetLastError());
+    return TCW_OK;
+}
+
+int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride, const double *t0,
+                      double *tau, const double *tau_lo, const double *tau_hi, int n_steps,
+                      double step0, double *twoF, int32_t *n_accept, void *stream) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (p->n_cand == 0) return TCW_OK;
+    if (!cols || !t0 || !tau || !twoF) return fail(TCW_ERR_INVALID, "tcw_refine_ascent: null argument");
+    if (n_steps < 0) return fail(TCW_ERR_INVALID, "n_steps must be >= 0");
+    if (!(step0 > 0.0) || !std::isfinite(step0)) return fail(TCW_ERR_INVALID, "step0 must be > 0");
+    if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
+    hipLaunchKernelGGL(refine_ascent_kernel, dim3((unsigned)p->n_items), dim3(RWG), p->lds_bytes,
+                       (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p, (const RItem *)p->d_items.p,
+                       (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols, col_stride,
+                       (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+    TCW_HIP_TRY(hipGetLastError());
+    return TCW_OK;
+}
+
+}  // extern "C"

@@ -259,3 +259,56 @@ def workload_exp_shared(T=256, n_sft=8192, n_t0=1024, n_tau=512, gap_frac=0.10,
         templates.append(multi_from_detectors(dets))
     win = grid_window(TRANSIENT_EXPONENTIAL, T_START, n_t0, n_tau)
     return templates, win
+
+
+class MultiDetectorBatch:
+    """Many templates of several detectors each, packed like the inputs of
+    BatchedTransientMaps: ``ts`` uint32 [total], ``cols`` float32 [7, total],
+    ``seg_off`` int64 [S+1] (detector segments), ``seg_begin`` int64 [T+1]
+    (segments of each template), ``windows`` (one per template)."""
+
+    def __init__(self, ts, cols, seg_off, seg_begin, windows, TAtom=TSFT, **info):
+        self.ts = ts
+        self.cols = cols
+        self.seg_off = seg_off
+        self.seg_begin = seg_begin
+        self.windows = windows
+        self.TAtom = TAtom
+        self.info = info
+
+    @property
+    def T(self):
+        return len(self.windows)
+
+
+def workload_exp_refine(T=4096, n_sft=2048, n_t0=256, n_tau=256, gap_frac=0.10,
+                        tau_range=(5.0, 150.0), seed_index=4):
+    """Exp window, T templates of 2 detectors (n_sft grid slots each,
+    independent 10% gaps), every template carrying an exponential transient
+    that starts on the 1800 s grid within the first n_t0 slots and decays with
+    tau ~ U[tau_range] * 1800 s (off the grid); n_t0 x n_tau map grid from the
+    data start in 1800 s steps, tau >= 3600 s.  The refinement workload: grid
+    maxima of these maps are the candidates."""
+    rng = rng_for(seed_index)
+    t_grid = T_START + np.arange(n_sft, dtype=np.float64) * TSFT
+    ts, cols, seg_off, seg_begin = [], [], [0], [0]
+    windows, t_on, tau_inj = [], [], []
+    for _t in range(T):
+        psi = rng.uniform(0, 2 * np.pi)
+        on = T_START + int(rng.integers(0, n_t0)) * TSFT
+        tau = float(rng.uniform(*tau_range)) * TSFT
+        for _X in range(2):
+            keep = gap_mask(rng, n_sft, gap_frac)
+            tt = t_grid[keep]
+            phi_a, phi_b = rng.uniform(0, 2 * np.pi, 2)
+            sig = exp_signal(tt, on, tau, psi)
+            ts.append(tt.astype(np.uint32))
+            cols.append(detector_atoms(rng, tt, phi_a, phi_b, sig))
+            seg_off.append(seg_off[-1] + tt.shape[0])
+        seg_begin.append(seg_begin[-1] + 2)
+        windows.append(grid_window(TRANSIENT_EXPONENTIAL, T_START, n_t0, n_tau))
+        t_on.append(on)
+        tau_inj.append(tau)
+    return MultiDetectorBatch(np.concatenate(ts), np.ascontiguousarray(np.concatenate(cols, 1)),
+                              np.asarray(seg_off, np.int64), np.asarray(seg_begin, np.int64),
+                              windows, t_on=np.asarray(t_on), tau_inj=np.asarray(tau_inj))

@@ -1,0 +1,280 @@
+"""Batched merge and exponential-window refinement on the GPU, against the
+oracle (oracle.merge_binned, oracle/refine_ref.py).
+
+Tolerances (FP64 sums on both sides; the kernel's weights come from one exp
+per lane and a geometric recurrence, the oracle's from np.exp per bin):
+  merge            bit-exact (float32 sums in detector order)
+  2F, lnBtSG       relative 1e-8
+  dF/dtau, dF/dS   relative 1e-6 of the candidate's largest component
+  dF/dt0           |.| <= 1e-9 |F| / tau  (zero in exact arithmetic)
+  atom gradients   float32 output: 2e-6 of the largest entry
+  2F at grid maxima vs the float32 map: relative 1e-4
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import refine_ref as rr
+from pyfstat_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TA = synth.TSFT
+
+
+def _device():
+    import torch
+
+    return torch.device("cuda", 0)
+
+
+def _merged(work):
+    import torch
+
+    from pyfstat_amd.refine import MergedAtoms
+
+    ts_d = torch.from_numpy(work.ts.view(np.int32)).to(_device())
+    cols_d = torch.from_numpy(work.cols).to(_device())
+    return MergedAtoms(ts_d, work.seg_off, work.seg_begin, work.TAtom, work.ts.size, _device(),
+                       cols_raw=cols_d), cols_d
+
+
+def _oracle_merge(work, t):
+    s0, s1 = int(work.seg_begin[t]), int(work.seg_begin[t + 1])
+    a, e = int(work.seg_off[s0]), int(work.seg_off[s1])
+    det_off = np.asarray(work.seg_off[s0:s1 + 1]) - a
+    return oracle.merge_binned(work.ts[a:e], work.cols[:, a:e], det_off, work.TAtom)
+
+
+@pytest.fixture(scope="module")
+def small(hip_ready):
+    work = synth.workload_exp_refine(T=5, n_sft=400, n_t0=40, n_tau=40, seed_index=40)
+    merged, cols_d = _merged(work)
+    return work, merged, cols_d
+
+
+def test_merge_batch_bit_exact(small):
+    work, merged, _ = small
+    cols = merged.cols.cpu().numpy()
+    ab = merged.atom_bin.cpu().numpy()
+    for t in range(work.T):
+        print(f"merge template {t}", flush=True)
+        ref, t0d = _oracle_merge(work, t)
+        assert int(merged.t_data[t]) == t0d
+        a, e = int(merged.m_off[t]), int(merged.m_off[t + 1])
+        assert e - a == ref.shape[1]
+        assert np.array_equal(cols[:, a:e], ref)
+        s0, s1 = int(work.seg_begin[t]), int(work.seg_begin[t + 1])
+        j0, j1 = int(work.seg_off[s0]), int(work.seg_off[s1])
+        want = a + (work.ts[j0:j1].astype(np.int64) - t0d) // TA
+        assert np.array_equal(ab[j0:j1], want)
+
+
+def test_merge_adjoint_gathers_bin_gradients(small):
+    import torch
+
+    work, merged, cols_d = small
+    raw = cols_d.clone().requires_grad_(True)
+    out = merged.merge(raw)
+    assert torch.equal(out, merged.cols)
+    R = torch.randn(out.shape, generator=torch.Generator().manual_seed(0)).to(out.device)
+    (out * R).sum().backward()
+    ab = merged.atom_bin
+    assert torch.equal(raw.grad, R[:, ab])
+
+
+def _problem(work, merged, per_template=4, skip=()):
+    from pyfstat_amd.batched import BatchedTransientMaps
+    from pyfstat_amd.refine import ExpRefineProblem, candidates_from_maps
+
+    b = BatchedTransientMaps(work.ts, work.cols, work.seg_off, work.seg_begin, work.windows,
+                             work.TAtom)
+    b.run()
+    ct, t0, tau, Fg = candidates_from_maps(b, per_template=per_template)
+    keep = ~np.isin(ct, np.asarray(skip, dtype=np.int64))
+    b.close()
+    # candidates_from_maps lists templates in order, so the problem keeps this order
+    return ExpRefineProblem(merged, ct[keep], t0[keep], tau[keep]), Fg[keep]
+
+
+def _oracle_eval(work, merged, prob, t0, tau):
+    cols = merged.cols.cpu().numpy().astype(np.float64)
+    return rr.batch_evaluate(cols, merged.m_off, merged.t_data, TA, prob.cand_tmpl, prob.k_lo,
+                             prob.k_hi, t0, tau)
+
+
+def test_forward_at_grid_maxima_equals_map(small):
+    work, merged, _ = small
+    prob, Fgrid = _problem(work, merged)
+    twoF, lnB, saved = prob.forward_raw(merged.cols, prob.t0_init, prob.tau_init)
+    twoF = twoF.cpu().numpy()
+    np.testing.assert_allclose(twoF, 2 * Fgrid, rtol=1e-4)
+    np.testing.assert_allclose(lnB.cpu().numpy(), np.log(70.0) + twoF / 2, rtol=1e-12)
+    F, dt0, dtau, g = _oracle_eval(work, merged, prob, prob.t0_init.cpu().numpy(),
+                                   prob.tau_init.cpu().numpy())
+    np.testing.assert_allclose(twoF, 2 * F, rtol=1e-8)
+
+
+def _check_forward(prob, merged, work, t0, tau):
+    import torch
+
+    t0_d = torch.from_numpy(t0).to(_device())
+    tau_d = torch.from_numpy(tau).to(_device())
+    twoF, lnB, saved = prob.forward_raw(merged.cols, t0_d, tau_d)
+    twoF = twoF.cpu().numpy()
+    saved = saved.cpu().numpy()
+    F, dt0, dtau, g = _oracle_eval(work, merged, prob, t0, tau)
+    np.testing.assert_allclose(twoF, 2 * F, rtol=1e-8)
+    np.testing.assert_allclose(saved[:, 9], F, rtol=1e-8)
+    scale = np.maximum(np.abs(g).max(axis=1), 1e-300)
+    assert np.all(np.abs(saved[:, 2:9] - g) <= 1e-6 * scale[:, None])
+    tau_scale = np.maximum(np.abs(dtau), 1e-12 * np.abs(F) / tau)
+    assert np.all(np.abs(saved[:, 1] - dtau) <= 1e-6 * tau_scale + 1e-12 * np.abs(F) / tau)
+    assert np.all(np.abs(saved[:, 0]) <= 1e-9 * np.abs(F) / tau)
+    return twoF, saved
+
+
+def test_forward_off_grid_vs_oracle(small):
+    work, merged, _ = small
+    prob, _ = _problem(work, merged)
+    rng = np.random.default_rng(2)
+    t0 = prob.t0_init.cpu().numpy() + rng.uniform(-0.4, 0.4, prob.n) * TA
+    tau = prob.tau_init.cpu().numpy() * rng.uniform(0.8, 1.25, prob.n)
+    print(f"off-grid forward: {prob.n} candidates", flush=True)
+    _check_forward(prob, merged, work, t0, tau)
+
+
+def test_gpu_finite_differences(small):
+    """Central differences of the GPU forward itself agree with its dF/dtau."""
+    import torch
+
+    work, merged, _ = small
+    prob, _ = _problem(work, merged)
+    tau = prob.tau_init * 1.05
+    h = 1e-5 * tau
+    F0, _, saved = prob.forward_raw(merged.cols, prob.t0_init, tau)
+    Fp = prob.forward_raw(merged.cols, prob.t0_init, tau + h, want_saved=False)[0]
+    Fm = prob.forward_raw(merged.cols, prob.t0_init, tau - h, want_saved=False)[0]
+    fd = ((Fp - Fm) / (2 * h) / 2).cpu().numpy()
+    an = saved[:, 1].cpu().numpy()
+    ok = saved[:, 9].cpu().numpy() != 2.0
+    err = np.abs(fd - an)[ok]
+    assert np.all(err <= 1e-5 * np.abs(an)[ok] + 1e-9 * (F0.cpu().numpy()[ok] / tau.cpu().numpy()[ok]))
+    # t0 differences vanish on the fixed support
+    d = torch.full_like(prob.t0_init, 300.0)
+    F1 = prob.forward_raw(merged.cols, prob.t0_init + d, tau, want_saved=False)[0]
+    np.testing.assert_allclose(F1.cpu().numpy(), F0.cpu().numpy(), rtol=1e-11)
+
+
+def test_autograd_backward_vs_oracle(small):
+    import torch
+
+    work, merged, _ = small
+    prob, _ = _problem(work, merged, skip=(2,))  # template 2 without candidates
+    assert prob.cand_off[3] == prob.cand_off[2]
+    rng = np.random.default_rng(4)
+    wa = rng.standard_normal(prob.n)
+    wb = rng.standard_normal(prob.n)
+    cols = merged.cols.clone().requires_grad_(True)
+    t0 = prob.t0_init.clone().requires_grad_(True)
+    tau = (prob.tau_init * 1.02).requires_grad_(True)
+    twoF, lnB = prob(cols, t0, tau)
+    loss = (torch.from_numpy(wa).to(_device()) * twoF).sum() + \
+        (torch.from_numpy(wb).to(_device()) * lnB).sum()
+    loss.backward()
+    F, dt0, dtau, g = _oracle_eval(work, merged, prob, t0.detach().cpu().numpy(),
+                                   tau.detach().cpu().numpy())
+    gF = 2 * wa + wb
+    np.testing.assert_allclose(tau.grad.cpu().numpy(), gF * dtau, rtol=1e-6,
+                               atol=1e-9 * np.abs(gF * F / tau.detach().cpu().numpy()).max())
+    assert np.all(np.abs(t0.grad.cpu().numpy()) <= 1e-8 * np.abs(gF * F) / prob.tau_init.cpu().numpy())
+    want = np.zeros(merged.cols.shape)
+    cols_h = merged.cols.cpu().numpy()
+    for i in range(prob.n):
+        t = int(prob.cand_tmpl[i])
+        a, e = int(merged.m_off[t]), int(merged.m_off[t + 1])
+        want[:, a:e] += rr.atom_grad((7, e - a), float(merged.t_data[t]), TA, int(prob.k_lo[i]),
+                                     int(prob.k_hi[i]), float(t0[i]), float(tau[i]), g[i], gF[i])
+    got = cols.grad.cpu().numpy()
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 2e-6 * scale
+    a, e = int(merged.m_off[2]), int(merged.m_off[3])
+    assert np.all(got[:, a:e] == 0)
+    assert cols_h.shape == got.shape
+
+
+def test_raw_atom_gradients_through_merge(small):
+    work, merged, cols_d = small
+    prob, _ = _problem(work, merged)
+    raw = cols_d.clone().requires_grad_(True)
+    m = merged.merge(raw)
+    mm = m.detach().clone().requires_grad_(True)
+    prob(mm, prob.t0_init, prob.tau_init)[0].sum().backward()
+    twoF = prob(m, prob.t0_init, prob.tau_init)[0]
+    twoF.sum().backward()
+    import torch
+
+    assert torch.equal(raw.grad, mm.grad[:, merged.atom_bin])
+
+
+def test_ascent_monotone_bounded_and_converged(small):
+    import torch
+
+    work, merged, _ = small
+    prob, _ = _problem(work, merged)
+    tau0 = prob.tau_init
+    lo = torch.clamp(tau0 - TA, min=float(TA))
+    hi = tau0 + TA
+    F_init = prob.forward_raw(merged.cols, prob.t0_init, tau0, want_saved=False)[0]
+    tau, twoF, nacc = prob.ascent(merged.cols, tau_lo=lo, tau_hi=hi, n_steps=20)
+    tau_h, twoF_h = tau.cpu().numpy(), twoF.cpu().numpy()
+    assert np.all(twoF_h >= F_init.cpu().numpy())
+    assert np.all(tau_h >= lo.cpu().numpy() * (1 - 1e-12)) and np.all(tau_h <= hi.cpu().numpy() * (1 + 1e-12))
+    again = prob.forward_raw(merged.cols, prob.t0_init, tau, want_saved=False)[0].cpu().numpy()
+    np.testing.assert_allclose(again, twoF_h, rtol=1e-12)
+    cols = merged.cols.cpu().numpy().astype(np.float64)
+    close = 0
+    for i in range(prob.n):
+        t = int(prob.cand_tmpl[i])
+        sub = cols[:, merged.m_off[t]:merged.m_off[t + 1]]
+        Fmax, _ = rr.scan_max_tau(sub, float(merged.t_data[t]), TA, int(prob.k_lo[i]),
+                                  int(prob.k_hi[i]), float(prob.t0_init[i]), float(lo[i]),
+                                  float(hi[i]), n=401)
+        close += int(twoF_h[i] >= 2 * Fmax - 1e-3 * abs(2 * Fmax))
+    print(f"ascent: {close}/{prob.n} within 1e-3 of the scanned maximum; "
+          f"accepted steps {nacc.cpu().numpy().tolist()}", flush=True)
+    assert close >= 0.8 * prob.n
+
+
+def test_global_path_long_support(hip_ready):
+    """Supports longer than the LDS staging capacity read atoms from HBM."""
+    import torch
+
+    from pyfstat_amd.refine import ExpRefineProblem
+
+    work = synth.workload_exp_refine(T=2, n_sft=3000, n_t0=8, n_tau=8, seed_index=41)
+    merged, _ = _merged(work)
+    ct = np.array([0, 0, 1, 1])
+    t0 = np.array([synth.T_START, synth.T_START + 3 * TA, synth.T_START, synth.T_START + TA])
+    tau = np.array([900 * TA, 20 * TA, 950 * TA, 5 * TA])
+    prob = ExpRefineProblem(merged, ct, t0, tau)
+    assert prob.support_lengths.max() > 2176
+    print(f"long supports: {prob.support_lengths.tolist()}", flush=True)
+    _check_forward(prob, merged, work, prob.t0_init.cpu().numpy(),
+                   prob.tau_init.cpu().numpy() * 1.01)
+    tau2, twoF, _ = prob.ascent(merged.cols, n_steps=5)
+    assert torch.all(twoF >= prob.forward_raw(merged.cols, prob.t0_init, prob.tau_init,
+                                              want_saved=False)[0])
+
+
+def test_problem_validation(small):
+    from pyfstat_amd.refine import ExpRefineProblem
+
+    work, merged, _ = small
+    with pytest.raises(ValueError):
+        ExpRefineProblem(merged, [0], [synth.T_START], [3600], k_lo=[5], k_hi=[4])
+    with pytest.raises(ValueError):
+        ExpRefineProblem(merged, [7], [synth.T_START], [3600])
+    with pytest.raises(ValueError):
+        ExpRefineProblem(merged, [0], [synth.T_START], [3600], k_lo=[0], k_hi=[10**6])
