@@ -376,6 +376,9 @@ def main():
     ap.add_argument("--templates", type=int, default=4096)
     ap.add_argument("--refine-steps", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in rows per work item instead of tuning them "
+                         "on this batch before the timed steps")
     ap.add_argument("--no-btsg", action="store_true")
     ap.add_argument("--plain-templates", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -417,6 +420,13 @@ def main():
     st0 = None
     log(f"workload ready in {time.perf_counter() - t_gen:.1f} s: T={b.T}, cells={cells:.3e}")
     log(f"plan: {cells} cells, scratch {b.scratch_bytes() / 2**30:.2f} GiB")
+    tuned = None
+    if not args.no_autotune and not args.profile_mode:
+        from pyfstat_amd import autotune
+
+        res = autotune.tune(b, reps=3, log=log)
+        tuned = {str(m): r["rows"] for m, r in res.items()}
+        log(f"autotune: rows per item by mode {tuned}")
 
     if args.profile_mode:
         for _ in range(args.warmup + args.steps):
@@ -511,6 +521,7 @@ def main():
             "binned_atoms": atoms_binned,
             "BtSG": BtSG,
             "hipgraph": use_graph,
+            "rows_per_item_by_mode": tuned,
             "arithmetic": "fp32 two-sum prefix scan, fp64 window differences/determinant/"
                           "normalisation, fp32 map",
         },

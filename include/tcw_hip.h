@@ -149,6 +149,11 @@ int tcw_batch_plan(tcw_handle *h, const tcw_batch_desc *desc, uint32_t flags,
 /* Device bytes of scratch the plan holds (prefix sums, partials, tables). */
 int64_t tcw_plan_scratch_bytes(const tcw_plan *p);
 
+/* Work items per kernel mode (items[0..3]: rect regular grid, rect generic,
+ * exp direct, exp on the atom grid); what the autotuner tunes.  No
+ * counterpart in the reference. */
+int tcw_plan_items(const tcw_plan *p, int64_t *items);
+
 /* Run a planned batch on device-resident atoms.  ts/cols follow the plan's
  * segment layout (cols is [TCW_NCOL][col_stride]); F_out is the concatenated
  * float32 output (F_off layout); stats_dev is a device array of T tcw_stats.

@@ -27,6 +27,8 @@ TCW_FLAG_GENERIC = 4
 MODE_RECT_FAST = 0
 MODE_RECT_GENERIC = 1
 MODE_EXP_DIRECT = 2
+MODE_EXP_FAST = 3
+MODE_COUNT = 4
 
 EXPORTED_SYMBOLS = (
     "tcw_version",
@@ -39,6 +41,7 @@ EXPORTED_SYMBOLS = (
     "tcw_map",
     "tcw_batch_plan",
     "tcw_plan_scratch_bytes",
+    "tcw_plan_items",
     "tcw_batch_run",
     "tcw_batch_run_timed",
     "tcw_plan_destroy",
@@ -172,6 +175,8 @@ def load():
     lib.tcw_batch_plan.argtypes = [vp, P(BatchDesc), ctypes.c_uint32, P(vp), vp]
     lib.tcw_plan_scratch_bytes.restype = ctypes.c_int64
     lib.tcw_plan_scratch_bytes.argtypes = [vp]
+    lib.tcw_plan_items.restype = ctypes.c_int
+    lib.tcw_plan_items.argtypes = [vp, vp]
     lib.tcw_batch_run.restype = ctypes.c_int
     lib.tcw_batch_run.argtypes = [vp, vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int, vp]
     lib.tcw_batch_run_timed.restype = ctypes.c_int

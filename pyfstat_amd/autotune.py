@@ -1,0 +1,136 @@
+"""Autotuner for the batched map kernels: rows (t0) per work item per kernel
+mode, chosen by timing the map kernels of the caller's own batch.
+
+A work item is up to ``rows`` t0 rows x 256 tau columns of one template.  More
+rows amortise the item's set-up (diagonal staging for the rect kernel, the
+chunk walk for the exp recurrence) and the per-item reductions; fewer rows give
+more items to spread over the 256 CUs and a smaller LDS footprint.  The best
+value depends on the batch's shapes, so it is measured: each candidate value
+is planned (``tcw_set_rows_per_item`` + ``BatchedTransientMaps.replan``) and
+the map kernels timed with HIP events (``run_timed``); the fastest is kept and
+cached per (device, mode, shape class).
+
+The reference has no counterpart (its kernels use fixed 1-D launches,
+pyfstat/tcw_fstat_map_funcs.py:862-870, 944-952).
+"""
+
+import json
+import math
+import os
+
+import numpy as np
+
+from . import _hiplib
+
+CANDIDATES = {
+    _hiplib.MODE_RECT_FAST: (64, 128, 192, 256, 384),
+    _hiplib.MODE_RECT_GENERIC: (16, 32, 64),
+    _hiplib.MODE_EXP_DIRECT: (16, 32, 64),
+    _hiplib.MODE_EXP_FAST: (32, 64, 96, 128),
+}
+
+_applied = {}
+
+
+def cache_path():
+    env = os.environ.get("PYFSTAT_AMD_TUNE_CACHE")
+    if env:
+        return env
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(base, "pyfstat_amd", "autotune.json")
+
+
+def _load_cache(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data if isinstance(data, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(path, data):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only costs re-tuning
+
+
+def _bucket(x):
+    return int(round(math.log2(max(float(x), 1.0))))
+
+
+def shape_class(b, mode, device_name=""):
+    """Cache key: device, mode, and log2 buckets of the templates' t0 rows,
+    tau columns and the mode's item count."""
+    shapes = np.array([b._shape(t) for t in range(b.T)], dtype=np.int64)
+    items = b.items_per_mode()[mode]
+    return (f"{device_name}|mode{mode}|rows{_bucket(np.median(shapes[:, 0]))}"
+            f"|cols{_bucket(np.median(shapes[:, 1]))}|items{_bucket(items)}")
+
+
+def set_rows(mode, rows):
+    lib = _hiplib.load()
+    _hiplib.check(lib.tcw_set_rows_per_item(int(mode), int(rows)), "tcw_set_rows_per_item")
+    _applied[int(mode)] = int(rows)
+
+
+def reset():
+    """Restore the built-in defaults for every mode."""
+    for mode in range(_hiplib.MODE_COUNT):
+        set_rows(mode, 0)
+    _applied.clear()
+
+
+def _time_maps(b, reps):
+    b.run_timed()  # warm: first launch after a replan
+    return min(b.run_timed()[1] for _ in range(reps))
+
+
+def tune(b, reps=3, candidates=None, use_cache=True, device_name=None, log=None):
+    """Pick rows per item for every kernel mode the batch uses; leaves the
+    winners set (and the batch re-planned with them).  Returns
+    {mode: {"rows": r, "maps_ms": t, "tried": {r: t}, "cached": bool}}."""
+    from .tcw_fstat_map_funcs import _hip_device_names, _HipState
+
+    cands = dict(CANDIDATES if candidates is None else candidates)
+    if device_name is None:
+        names = _hip_device_names()
+        dev = _HipState.device or 0
+        device_name = names[dev] if dev < len(names) else ""
+    path = cache_path()
+    cache = _load_cache(path) if use_cache else {}
+    items = b.items_per_mode()
+    result = {}
+    changed = False
+    for mode in range(_hiplib.MODE_COUNT):
+        if items[mode] == 0 or mode not in cands:
+            continue
+        key = shape_class(b, mode, device_name)
+        if use_cache and key in cache:
+            set_rows(mode, cache[key]["rows"])
+            result[mode] = dict(cache[key], cached=True)
+            changed = True
+            continue
+        tried = {}
+        for rows in cands[mode]:
+            set_rows(mode, rows)
+            b.replan()
+            tried[rows] = _time_maps(b, reps)
+            if log:
+                log(f"autotune mode {mode}: rows {rows} -> maps {tried[rows]:.4f} ms")
+        best = min(tried, key=tried.get)
+        set_rows(mode, best)
+        changed = True
+        result[mode] = {"rows": int(best), "maps_ms": float(tried[best]),
+                        "tried": {str(k): float(v) for k, v in tried.items()}, "cached": False}
+        cache[key] = {k: v for k, v in result[mode].items() if k != "cached"}
+    if changed:
+        b.replan()
+    if use_cache:
+        _save_cache(path, cache)
+    return result

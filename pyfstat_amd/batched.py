@@ -80,18 +80,36 @@ class BatchedTransientMaps:
         desc.seg_tfirst = self.seg_tfirst.ctypes.data
         desc.seg_tlast = self.seg_tlast.ctypes.data
         desc.windows = ctypes.addressof(self.win_arr)
+        self._desc = desc
         self.F_off = np.zeros(self.T + 1, dtype=np.int64)
-        plan = ctypes.c_void_p()
-        _hiplib.check(self.lib.tcw_batch_plan(self.handle, ctypes.byref(desc), self.flags,
-                                              ctypes.byref(plan), self.F_off.ctypes.data),
-                      "tcw_batch_plan")
-        self.plan = plan
+        self.plan = None
+        self._make_plan()
         self.ts_d = torch.from_numpy(ts.view(np.int32)).to(self.device)
         self.cols_d = torch.from_numpy(cols).to(self.device)
         self.col_stride = cols.shape[1]
         self.F_d = torch.empty(int(self.F_off[-1]), dtype=torch.float32, device=self.device)
         self.stats_d = torch.empty(self.T * STATS_DTYPE.itemsize, dtype=torch.uint8,
                                    device=self.device)
+
+    def _make_plan(self):
+        plan = ctypes.c_void_p()
+        _hiplib.check(self.lib.tcw_batch_plan(self.handle, ctypes.byref(self._desc), self.flags,
+                                              ctypes.byref(plan), self.F_off.ctypes.data),
+                      "tcw_batch_plan")
+        self.plan = plan
+
+    def replan(self):
+        """Rebuild the plan (work items, scratch, graph) with the current
+        rows-per-item settings; atoms and outputs stay where they are."""
+        self.close()
+        self._make_plan()
+
+    def items_per_mode(self):
+        """Work items per kernel mode (rect regular, rect generic, exp direct,
+        exp on the atom grid)."""
+        out = np.zeros(_hiplib.MODE_COUNT, dtype=np.int64)
+        _hiplib.check(self.lib.tcw_plan_items(self.plan, out.ctypes.data), "tcw_plan_items")
+        return out
 
     # ------------------------------------------------------------------
     @classmethod

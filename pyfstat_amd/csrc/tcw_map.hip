@@ -1832,6 +1832,12 @@ int64_t tcw_plan_scratch_bytes(const tcw_plan *p) {
                      p->d_etab.cap);
 }
 
+int tcw_plan_items(const tcw_plan *p, int64_t *items) {
+    if (!p || !items) return fail(TCW_ERR_INVALID, "tcw_plan_items: null argument");
+    for (int md = 0; md < MODE_COUNT; md++) items[md] = p->mode_begin[md + 1] - p->mode_begin[md];
+    return TCW_OK;
+}
+
 int tcw_batch_run(tcw_handle *h, tcw_plan *p, const uint32_t *ts, const float *cols,
                   int64_t col_stride, float *F_out, tcw_stats *stats_dev, int use_graph,
                   void *stream) {
