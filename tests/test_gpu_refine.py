@@ -229,7 +229,9 @@ def test_ascent_monotone_bounded_and_converged(small):
     F_init = prob.forward_raw(merged.cols, prob.t0_init, tau0, want_saved=False)[0]
     tau, twoF, nacc = prob.ascent(merged.cols, tau_lo=lo, tau_hi=hi, n_steps=20)
     tau_h, twoF_h = tau.cpu().numpy(), twoF.cpu().numpy()
-    assert np.all(twoF_h >= F_init.cpu().numpy())
+    # never below the start (the forward and ascent kernels share the evaluation;
+    # allow last-bit differences of separately compiled copies)
+    assert np.all(twoF_h >= F_init.cpu().numpy() * (1 - 1e-12))
     assert np.all(tau_h >= lo.cpu().numpy() * (1 - 1e-12)) and np.all(tau_h <= hi.cpu().numpy() * (1 + 1e-12))
     again = prob.forward_raw(merged.cols, prob.t0_init, tau, want_saved=False)[0].cpu().numpy()
     np.testing.assert_allclose(again, twoF_h, rtol=1e-12)
@@ -265,7 +267,7 @@ def test_global_path_long_support(hip_ready):
                    prob.tau_init.cpu().numpy() * 1.01)
     tau2, twoF, _ = prob.ascent(merged.cols, n_steps=5)
     assert torch.all(twoF >= prob.forward_raw(merged.cols, prob.t0_init, prob.tau_init,
-                                              want_saved=False)[0])
+                                              want_saved=False)[0] * (1 - 1e-12))
 
 
 def test_problem_validation(small):
