@@ -1,5 +1,5 @@
 """Autotuner for the batched map kernels: rows (t0) per work item per kernel
-mode, chosen by timing the map kernels of the caller's own batch.
+mode, chosen by timing the step's kernels on the caller's own batch.
 
 A work item is up to ``rows`` t0 rows x 256 tau columns of one template.  More
 rows amortise the item's set-up (diagonal staging for the rect kernel, the
@@ -7,8 +7,8 @@ chunk walk for the exp recurrence) and the per-item reductions; fewer rows give
 more items to spread over the 256 CUs and a smaller LDS footprint.  The best
 value depends on the batch's shapes, so it is measured: each candidate value
 is planned (``tcw_set_rows_per_item`` + ``BatchedTransientMaps.replan``) and
-the map kernels timed with HIP events (``run_timed``); the fastest is kept and
-cached per (device, mode, shape class).
+the step's kernels timed with HIP events (``run_timed``); the fastest is kept
+and cached per (device, mode, shape class).
 
 The reference has no counterpart (its kernels use fixed 1-D launches,
 pyfstat/tcw_fstat_map_funcs.py:862-870, 944-952).
@@ -65,12 +65,13 @@ def _bucket(x):
 
 
 def shape_class(b, mode, device_name=""):
-    """Cache key: device, mode, and log2 buckets of the templates' t0 rows,
-    tau columns and the mode's item count."""
+    """Cache key: device, mode, and log2 buckets of the template count and of
+    the templates' median t0 rows and tau columns (none of which depends on
+    the rows-per-item setting being tuned)."""
     shapes = np.array([b._shape(t) for t in range(b.T)], dtype=np.int64)
-    items = b.items_per_mode()[mode]
-    return (f"{device_name}|mode{mode}|rows{_bucket(np.median(shapes[:, 0]))}"
-            f"|cols{_bucket(np.median(shapes[:, 1]))}|items{_bucket(items)}")
+    version = _hiplib.load().tcw_version().decode(errors="replace")
+    return (f"{version}|{device_name}|mode{mode}|T{_bucket(b.T)}"
+            f"|rows{_bucket(np.median(shapes[:, 0]))}|cols{_bucket(np.median(shapes[:, 1]))}")
 
 
 def set_rows(mode, rows):
@@ -86,15 +87,18 @@ def reset():
     _applied.clear()
 
 
-def _time_maps(b, reps):
+def _time_step(b, reps):
+    """Best of reps: stage (merge, prefix or exp chunk scan) + maps +
+    finalize, each timed with HIP events; rows per item moves work between
+    the exp chunk scan and the map kernel, so the whole step is what counts."""
     b.run_timed()  # warm: first launch after a replan
-    return min(b.run_timed()[1] for _ in range(reps))
+    return min(sum(b.run_timed()) for _ in range(reps))
 
 
 def tune(b, reps=3, candidates=None, use_cache=True, device_name=None, log=None):
     """Pick rows per item for every kernel mode the batch uses; leaves the
     winners set (and the batch re-planned with them).  Returns
-    {mode: {"rows": r, "maps_ms": t, "tried": {r: t}, "cached": bool}}."""
+    {mode: {"rows": r, "step_ms": t, "tried": {r: t}, "cached": bool}}."""
     from .tcw_fstat_map_funcs import _hip_device_names, _HipState
 
     cands = dict(CANDIDATES if candidates is None else candidates)
@@ -120,13 +124,13 @@ def tune(b, reps=3, candidates=None, use_cache=True, device_name=None, log=None)
         for rows in cands[mode]:
             set_rows(mode, rows)
             b.replan()
-            tried[rows] = _time_maps(b, reps)
+            tried[rows] = _time_step(b, reps)
             if log:
-                log(f"autotune mode {mode}: rows {rows} -> maps {tried[rows]:.4f} ms")
+                log(f"autotune mode {mode}: rows {rows} -> step {tried[rows]:.4f} ms")
         best = min(tried, key=tried.get)
         set_rows(mode, best)
         changed = True
-        result[mode] = {"rows": int(best), "maps_ms": float(tried[best]),
+        result[mode] = {"rows": int(best), "step_ms": float(tried[best]),
                         "tried": {str(k): float(v) for k, v in tried.items()}, "cached": False}
         cache[key] = {k: v for k, v in result[mode].items() if k != "cached"}
     if changed:
