@@ -301,18 +301,18 @@ def run_refine(args):
     if not args.no_cpu_baseline:
         # the FP64 numpy reference, one core: evaluations (2F + derivatives) at
         # the candidates' start points, repeated until about --cpu-seconds pass
-        sample = rng.choice(prob.n, size=min(4096, prob.n), replace=False)
+        sample = rng.permutation(prob.n)
+        tau0_h = prob.tau_init.cpu().numpy()
         done, bins = 0, 0
         c0 = time.perf_counter()
-        for i in sample:
+        while time.perf_counter() - c0 < args.cpu_seconds:
+            i = int(sample[done % prob.n])
             t = int(prob.cand_tmpl[i])
             sub = cols_h[:, merged.m_off[t]:merged.m_off[t + 1]]
             rr.evaluate(sub, float(merged.t_data[t]), TA, int(prob.k_lo[i]), int(prob.k_hi[i]),
-                        float(t0_h[i]), float(prob.tau_init[i]))
+                        float(t0_h[i]), float(tau0_h[i]))
             done += 1
             bins += int(L[i])
-            if time.perf_counter() - c0 > args.cpu_seconds:
-                break
         dt = time.perf_counter() - c0
         cpu = {"value": done / dt, "unit": "candidate-steps/s", "cores": 1, "kind": "port",
                "sample": f"{done} random candidates ({bins} support bins, {dt:.1f} s), FP64 "
@@ -379,6 +379,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in rows per work item instead of tuning them "
                          "on this batch before the timed steps")
+    ap.add_argument("--rows", nargs="*", default=None, metavar="MODE:ROWS",
+                    help="fixed rows per work item per kernel mode (e.g. 0:192 3:96); "
+                         "implies --no-autotune")
     ap.add_argument("--no-btsg", action="store_true")
     ap.add_argument("--plain-templates", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -421,7 +424,16 @@ def main():
     log(f"workload ready in {time.perf_counter() - t_gen:.1f} s: T={b.T}, cells={cells:.3e}")
     log(f"plan: {cells} cells, scratch {b.scratch_bytes() / 2**30:.2f} GiB")
     tuned = None
-    if not args.no_autotune and not args.profile_mode:
+    if args.rows:
+        from pyfstat_amd import autotune
+
+        for item in args.rows:
+            mode, rows = (int(x) for x in item.split(":"))
+            autotune.set_rows(mode, rows)
+        b.replan()
+        tuned = {item.split(":")[0]: int(item.split(":")[1]) for item in args.rows}
+        log(f"rows per item by mode (fixed): {tuned}")
+    elif not args.no_autotune and not args.profile_mode:
         from pyfstat_amd import autotune
 
         res = autotune.tune(b, reps=3, log=log)
