@@ -978,6 +978,20 @@ __device__ __forceinline__ void expf_step(const ExpWalk &wk, const ExpCol &ec, i
     }
 }
 
+// The common case of one row step = one atom step (dt0 == TAtom) with the
+// atoms staged in LDS: jh / jt are the LDS slots of the entering atom
+// (k - 1) and of the leaving atom (k - 1 + L), both falling by one per row.
+__device__ __forceinline__ void expf_step1(const ExpWalk &wk, const ExpCol &ec, int jh, int jt,
+                                           double (&E)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int g = c < 3 ? 0 : 1;
+        const double xh = wk.lds[c * wk.stride + jh];
+        const double xt = wk.lds[c * wk.stride + jt];
+        E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh));
+    }
+}
+
 // Etab layout per template: [segment][component][tau column]
 __device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, int s, int c) {
     return etab + td.etab_off + ((int64_t)s * NC + c) * td.N_tau;
@@ -1005,9 +1019,19 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
 #pragma unroll
     for (int c = 0; c < NC; c++) E[c] = 0.0;
     int k = td.k00 + td.a * r_top;
-    for (int m = r_top; m > r_bot; m--) {
-        expf_step(wk, ec, td.a, k, E);
-        k -= td.a;
+    if (wk.in_lds && td.a == 1) {  // block-uniform
+        int jh = (int)(k - 1 - wk.head_lo);
+        int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
+        for (int m = r_top; m > r_bot; m--) {
+            expf_step1(wk, ec, jh, jt, E);
+            jh--;
+            jt--;
+        }
+    } else {
+        for (int m = r_top; m > r_bot; m--) {
+            expf_step(wk, ec, td.a, k, E);
+            k -= td.a;
+        }
     }
     if (valid) {
 #pragma unroll
@@ -1084,17 +1108,37 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     float *__restrict__ Ft = Fout + td.F_off;
     const int colidx[1] = {n};
     int k = td.k00 + td.a * r_top;
-    for (int m = r_top; m >= row0; m--) {
-        if (m <= m_hi) {  // wave-uniform: rows past the map are only walked
-            const float f = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
-            if (valid) Ft[(int64_t)m * N_tau + n] = f;
-            if (STRICT) single |= valid && (k >= (int)td.N - 1);
-            const float F1[1][1] = {{valid ? f : -INFINITY}};
-            red.template add<1>(F1, m, colidx, N_tau, lane, m - row0, sh_rowM[wave],
-                                sh_rowS[wave]);
+    const bool step1 = wk.in_lds && td.a == 1;  // block-uniform
+    int jh = (int)(k - 1 - wk.head_lo);
+    int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
+    // rows in groups of 4 (RB is a multiple of 16), reduced together
+    for (int g = r_top; g >= row0; g -= 4) {
+        float F4[4][1];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int m = g - j;
+            float f = -INFINITY;
+            if (m <= m_hi) {  // wave-uniform: rows past the map are only walked
+                const float fv = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
+                if (valid) Ft[(int64_t)m * N_tau + n] = fv;
+                if (STRICT) single |= valid && (k >= (int)td.N - 1);
+                f = valid ? fv : -INFINITY;
+            }
+            F4[3 - j][0] = f;  // F4[r] is row g - 3 + r
+            if (m > row0) {
+                if (step1) {
+                    expf_step1(wk, ec, jh, jt, E);
+                    jh--;
+                    jt--;
+                } else {
+                    expf_step(wk, ec, td.a, k, E);
+                }
+            }
+            k -= td.a;
         }
-        if (m > row0) expf_step(wk, ec, td.a, k, E);
-        k -= td.a;
+        if (g - 3 <= m_hi)
+            red.template add<4>(F4, g - 3, colidx, N_tau, lane, g - 3 - row0, sh_rowM[wave],
+                                sh_rowS[wave]);
     }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     // combine: each column belongs to one lane; each row to all 4 waves
