@@ -35,7 +35,9 @@ namespace {
 
 constexpr int NC = TCW_NCOL;
 constexpr int RWG = 256;             // 4 waves
-constexpr int CAND_PER_ITEM = 8;     // candidates per work item (2 per wave)
+constexpr int GROUP = 16;            // lanes per candidate
+constexpr int GROUPS_PER_WAVE = 64 / GROUP;
+constexpr int CAND_PER_ITEM = (RWG / 64) * GROUPS_PER_WAVE;  // 16: 4 per wave
 constexpr int LDS_ATOMS_CAP = 2176;  // staged support bins: 7*4*2176 = 60928 B
 constexpr int BWD_BINS = 256;        // bins per backward block
 constexpr double COND_MAX = 1e4;
@@ -70,50 +72,43 @@ struct Eval {
     double g[NC];
 };
 
-__device__ __forceinline__ double readlane_d(double x, int l) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// Sum 16 per-lane values over the wave by recursive halving: each round a lane
-// hands the half of its values it does not keep to its partner, so 8+4+2+1
-// exchanges replace 16*6.  Afterwards lane L holds the total of value (L>>2)&15.
-__device__ __forceinline__ double wave_transpose_sum16(double (&v)[16], int lane) {
+// Sum 16 per-lane values over the 16 lanes of a group by recursive halving:
+// each round a lane hands the half of its values it does not keep to its
+// partner, so 8+4+2+1 exchanges replace 16*4.  Afterwards lane gl of the
+// group holds the group total of value gl.
+__device__ __forceinline__ double group_transpose_sum16(double (&v)[16], int gl) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const int off = 32 >> r;
-        const int h = 8 >> r;
-        const bool up = (lane & off) != 0;
+        const int off = 8 >> r;
+        const bool up = (gl & off) != 0;
 #pragma unroll
-        for (int i = 0; i < h; i++) {
-            const double send = up ? v[i] : v[i + h];
-            const double keep = up ? v[i + h] : v[i];
+        for (int i = 0; i < off; i++) {
+            const double send = up ? v[i] : v[i + off];
+            const double keep = up ? v[i + off] : v[i];
             v[i] = keep + __shfl_xor(send, off, 64);
         }
     }
-    double x = v[0];
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
-    return x;
+    return v[0];
 }
 
-// F and its derivatives for one candidate, evaluated by one wave.  Atom (c, k)
-// of the template is src[c*sstride + k - sbase].  Every lane returns the same
-// values (wave-uniform control flow for the callers).
-__device__ Eval eval_candidate(const float *src, int64_t sstride, int sbase, int kb, int ke,
-                               double t_data, double TA, double t0, double tau, int lane) {
+// F and its derivatives for one candidate per 16-lane group (4 candidates per
+// wave).  Atom (c, k) of the template is src[c*sstride + k - sbase]; an empty
+// support (kb > ke) gives the F = 2 branch.  Every lane of a group returns
+// its candidate's values.
+__device__ Eval eval_group(const float *src, int64_t sstride, int sbase, int kb, int ke,
+                           double t_data, double TA, double t0, double tau, int lane) {
+    const int gl = lane & (GROUP - 1);
+    const int gbase = lane & ~(GROUP - 1);
     double v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = 0.0;
     const double inv_tau = 1.0 / tau;
-    const int k0 = kb + lane;
+    const int k0 = kb + gl;
     double u = fma((double)k0, TA, t_data - t0);  // t_k - t0
     double w = exp(-u * inv_tau);
-    const double r64 = exp(-64.0 * TA * inv_tau);
-    const double du = 64.0 * TA;
-    for (int k = k0; k <= ke; k += 64) {
+    const double rg = exp(-(double)GROUP * TA * inv_tau);
+    const double du = (double)GROUP * TA;
+    for (int k = k0; k <= ke; k += GROUP) {
         const float *a = src + (k - sbase);
         const double w2 = w * w;
 #pragma unroll
@@ -122,32 +117,26 @@ __device__ Eval eval_candidate(const float *src, int64_t sstride, int sbase, int
             v[c] += x;
             v[NC + c] = fma(u, x, v[NC + c]);
         }
-        w *= r64;
+        w *= rg;
         u += du;
     }
-    const double red = wave_transpose_sum16(v, lane);
+    const double red = group_transpose_sum16(v, gl);
     double s[2 * NC];
 #pragma unroll
-    for (int j = 0; j < 2 * NC; j++) s[j] = readlane_d(red, 4 * j);
+    for (int j = 0; j < 2 * NC; j++) s[j] = __shfl(red, gbase + j, 64);
 
     Eval e;
     const double A = s[0], B = s[1], C = s[2];
     const double far = s[3], fai = s[4], fbr = s[5], fbi = s[6];
     const double D = fma(A, B, -(C * C));
     const double T = A + B;
-    if (!((T > 0.0) && (T * T < QCOND * D))) {
-        e.F = 2.0;
-        e.dFdt0 = e.dFdtau = 0.0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) e.g[c] = 0.0;
-        return e;
-    }
-    const double iD = 1.0 / D;
+    const bool ok = (T > 0.0) && (T * T < QCOND * D);
+    const double iD = ok ? 1.0 / D : 0.0;
     const double p = fma(far, far, fai * fai);
     const double q = fma(fbr, fbr, fbi * fbi);
     const double rr = fma(far, fbr, fai * fbi);
     const double F = fma(B, p, fma(A, q, (-2.0 * C) * rr)) * iD;
-    e.F = F;
+    e.F = ok ? F : 2.0;
     e.g[0] = (q - F * B) * iD;
     e.g[1] = (p - F * A) * iD;
     e.g[2] = 2.0 * (F * C - rr) * iD;
@@ -187,6 +176,26 @@ __device__ __forceinline__ const float *stage_support(const RItem &it, const RTm
     return cols + tp.m_off;
 }
 
+// Candidate slot of a lane: group (lane / 16) of wave w handles item
+// candidate j = w*4 + group; lanes without a candidate get an empty support.
+struct Slot {
+    int64_t ci;
+    bool has;
+    int kb, ke;
+};
+
+__device__ __forceinline__ Slot lane_slot(const RItem &it, const int32_t *k_lo,
+                                          const int32_t *k_hi) {
+    const int lane = threadIdx.x & 63;
+    const int j = (threadIdx.x >> 6) * GROUPS_PER_WAVE + (lane >> 4);
+    Slot sl;
+    sl.has = j < it.ncand;
+    sl.ci = it.cbeg + (sl.has ? j : 0);
+    sl.kb = sl.has ? k_lo[sl.ci] : 1;
+    sl.ke = sl.has ? k_hi[sl.ci] : 0;
+    return sl;
+}
+
 __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
@@ -198,24 +207,25 @@ __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     int64_t sstride;
     int sbase;
     const float *src = stage_support(it, tp, cols, cstride, lds, sstride, sbase);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = wave; j < it.ncand; j += RWG / 64) {
-        const int64_t ci = it.cbeg + j;
-        const Eval e = eval_candidate(src, sstride, sbase, k_lo[ci], k_hi[ci], (double)tp.t_data,
-                                      TA, t0v[ci], tauv[ci], lane);
-        if (lane == 0) {
-            twoF[ci] = 2.0 * e.F;
-            if (lnB) lnB[ci] = LN70 + e.F;
-        }
-        if (saved && lane < NSAVED) {
-            double x = e.F;
-            if (lane == 0) x = e.dFdt0;
-            if (lane == 1) x = e.dFdtau;
+    const int lane = threadIdx.x & 63, gl = lane & (GROUP - 1);
+    const Slot sl = lane_slot(it, k_lo, k_hi);
+    const double t0 = sl.has ? t0v[sl.ci] : 0.0;
+    const double tau = sl.has ? tauv[sl.ci] : 1.0;
+    const Eval e = eval_group(src, sstride, sbase, sl.kb, sl.ke, (double)tp.t_data, TA, t0, tau,
+                              lane);
+    if (!sl.has) return;
+    if (gl == 0) {
+        twoF[sl.ci] = 2.0 * e.F;
+        if (lnB) lnB[sl.ci] = LN70 + e.F;
+    }
+    if (saved && gl < NSAVED) {
+        double x = e.F;
+        if (gl == 0) x = e.dFdt0;
+        if (gl == 1) x = e.dFdtau;
 #pragma unroll
-            for (int c = 0; c < NC; c++)
-                if (lane == 2 + c) x = e.g[c];
-            saved[ci * NSAVED + lane] = x;
-        }
+        for (int c = 0; c < NC; c++)
+            if (gl == 2 + c) x = e.g[c];
+        saved[sl.ci * NSAVED + gl] = x;
     }
 }
 
@@ -231,49 +241,42 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     int64_t sstride;
     int sbase;
     const float *src = stage_support(it, tp, cols, cstride, lds, sstride, sbase);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = wave; j < it.ncand; j += RWG / 64) {
-        const int64_t ci = it.cbeg + j;
-        const int kb = k_lo[ci], ke = k_hi[ci];
-        const double t0 = t0v[ci];
-        double tau = tauv[ci];
-        int acc = 0;
-        double F2 = NAN;
-        if (tau > 0.0) {
-            const double lo = tau_lo ? log(tau_lo[ci]) : -INFINITY;
-            const double hi = tau_hi ? log(tau_hi[ci]) : INFINITY;
-            Eval e = eval_candidate(src, sstride, sbase, kb, ke, (double)tp.t_data, TA, t0, tau,
-                                    lane);
-            double s = log(tau);
-            double g = tau * e.dFdtau;  // dF/dlog(tau)
-            double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
-            for (int step = 0; step < n_steps; step++) {
-                const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
-                if (!(sn != s)) {  // no move possible with this step size
-                    eta *= 0.5;
-                    continue;
-                }
-                const double taun = exp(sn);
-                const Eval en = eval_candidate(src, sstride, sbase, kb, ke, (double)tp.t_data, TA,
-                                               t0, taun, lane);
-                if (en.F > e.F) {
-                    s = sn;
-                    tau = taun;
-                    e = en;
-                    g = tau * e.dFdtau;
-                    eta *= 1.5;
-                    acc++;
-                } else {
-                    eta *= 0.5;
-                }
-            }
-            F2 = 2.0 * e.F;
+    const int lane = threadIdx.x & 63, gl = lane & (GROUP - 1);
+    const Slot sl = lane_slot(it, k_lo, k_hi);
+    const double t0 = sl.has ? t0v[sl.ci] : 0.0;
+    const double tau_in = sl.has ? tauv[sl.ci] : 1.0;
+    const bool live = sl.has && tau_in > 0.0;
+    double tau = live ? tau_in : 1.0;
+    const double lo = (live && tau_lo) ? log(tau_lo[sl.ci]) : -INFINITY;
+    const double hi = (live && tau_hi) ? log(tau_hi[sl.ci]) : INFINITY;
+    const int kb = live ? sl.kb : 1, ke = live ? sl.ke : 0;
+    Eval e = eval_group(src, sstride, sbase, kb, ke, (double)tp.t_data, TA, t0, tau, lane);
+    double s = log(tau);
+    double g = tau * e.dFdtau;  // dF/dlog(tau)
+    double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
+    int acc = 0;
+    // every group evaluates every step (no divergent skips); a proposal that
+    // cannot move (zero gradient or clamped) is rejected like a worse one
+    for (int step = 0; step < n_steps; step++) {
+        const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
+        const bool moved = sn != s;
+        const double taun = moved ? exp(sn) : tau;
+        const Eval en = eval_group(src, sstride, sbase, kb, ke, (double)tp.t_data, TA, t0, taun,
+                                   lane);
+        const bool accept = moved && en.F > e.F;
+        if (accept) {
+            s = sn;
+            tau = taun;
+            e = en;
+            g = tau * e.dFdtau;
+            acc++;
         }
-        if (lane == 0) {
-            tauv[ci] = tau;
-            twoF[ci] = F2;
-            if (n_accept) n_accept[ci] = acc;
-        }
+        eta *= accept ? 1.5 : 0.5;
+    }
+    if (sl.has && gl == 0) {
+        tauv[sl.ci] = live ? tau : tau_in;
+        twoF[sl.ci] = live ? 2.0 * e.F : NAN;
+        if (n_accept) n_accept[sl.ci] = acc;
     }
 }
 

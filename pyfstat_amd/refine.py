@@ -214,9 +214,9 @@ class ExpRefineProblem:
     t0, tau : [n] the cells the supports are taken from (integer seconds), or
         pass ``k_lo``/``k_hi`` explicitly.
 
-    Candidates are kept sorted by template (stable); ``order[i]`` is the
-    caller's index of candidate i, and every array this class takes or returns
-    is in that sorted order.
+    Candidates are kept sorted by template and, within a template, by support
+    length (stable); ``order[i]`` is the caller's index of candidate i, and
+    every array this class takes or returns is in that sorted order.
     """
 
     def __init__(self, merged, cand_tmpl, t0, tau, k_lo=None, k_hi=None):
@@ -227,19 +227,22 @@ class ExpRefineProblem:
         cand_tmpl = np.asarray(cand_tmpl, dtype=np.int64)
         if cand_tmpl.size and (cand_tmpl.min() < 0 or cand_tmpl.max() >= merged.T):
             raise ValueError("candidate template index out of range")
-        self.order = np.argsort(cand_tmpl, kind="stable")
-        self.cand_tmpl = cand_tmpl[self.order]
-        t0 = np.asarray(t0, dtype=np.float64)[self.order]
-        tau = np.asarray(tau, dtype=np.float64)[self.order]
+        t0 = np.asarray(t0, dtype=np.float64)
+        tau = np.asarray(tau, dtype=np.float64)
         if k_lo is None:
             k_lo, k_hi = exp_window_support(np.rint(t0), np.rint(tau),
-                                            merged.t_data[self.cand_tmpl].astype(np.int64),
-                                            merged.N[self.cand_tmpl], merged.TAtom)
-        else:
-            k_lo = np.asarray(k_lo, dtype=np.int32)[self.order]
-            k_hi = np.asarray(k_hi, dtype=np.int32)[self.order]
-        self.k_lo = np.ascontiguousarray(k_lo, dtype=np.int32)
-        self.k_hi = np.ascontiguousarray(k_hi, dtype=np.int32)
+                                            merged.t_data[cand_tmpl].astype(np.int64),
+                                            merged.N[cand_tmpl], merged.TAtom)
+        k_lo = np.asarray(k_lo, dtype=np.int32)
+        k_hi = np.asarray(k_hi, dtype=np.int32)
+        # by template, then by support length: the 16-lane groups of a wave
+        # then walk supports of similar length
+        self.order = np.lexsort((k_hi.astype(np.int64) - k_lo, cand_tmpl))
+        self.cand_tmpl = cand_tmpl[self.order]
+        t0 = t0[self.order]
+        tau = tau[self.order]
+        self.k_lo = np.ascontiguousarray(k_lo[self.order], dtype=np.int32)
+        self.k_hi = np.ascontiguousarray(k_hi[self.order], dtype=np.int32)
         self.n = int(self.cand_tmpl.size)
         self.cand_off = np.zeros(merged.T + 1, dtype=np.int64)
         np.add.at(self.cand_off, self.cand_tmpl + 1, 1)

@@ -94,8 +94,8 @@ def _problem(work, merged, per_template=4, skip=()):
     ct, t0, tau, Fg = candidates_from_maps(b, per_template=per_template)
     keep = ~np.isin(ct, np.asarray(skip, dtype=np.int64))
     b.close()
-    # candidates_from_maps lists templates in order, so the problem keeps this order
-    return ExpRefineProblem(merged, ct[keep], t0[keep], tau[keep]), Fg[keep]
+    prob = ExpRefineProblem(merged, ct[keep], t0[keep], tau[keep])
+    return prob, Fg[keep][prob.order]
 
 
 def _oracle_eval(work, merged, prob, t0, tau):
