@@ -72,59 +72,76 @@ struct Eval {
     double g[NC];
 };
 
-// Sum 16 per-lane values over the 16 lanes of a group by recursive halving:
-// each round a lane hands the half of its values it does not keep to its
-// partner, so 8+4+2+1 exchanges replace 16*4.  Afterwards lane gl of the
-// group holds the group total of value gl.
-__device__ __forceinline__ double group_transpose_sum16(double (&v)[16], int gl) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int off = 8 >> r;
-        const bool up = (gl & off) != 0;
-#pragma unroll
-        for (int i = 0; i < off; i++) {
-            const double send = up ? v[i] : v[i + off];
-            const double keep = up ? v[i + off] : v[i];
-            v[i] = keep + __shfl_xor(send, off, 64);
-        }
-    }
-    return v[0];
+// DPP exchanges within a 16-lane row (gfx9 encodings): each pairs every lane
+// with exactly one partner, so a + b on one side and b + a on the other give
+// bit-identical sums; mirror, half-mirror, xor 2, xor 1 together cover the
+// 16 lanes of the group.
+constexpr int DPP_ROW_MIRROR = 0x140;
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_QUAD_XOR2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1,0,3,2]
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_pair(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// F and its derivatives for one candidate per 16-lane group (4 candidates per
-// wave).  Atom (c, k) of the template is src[c*sstride + k - sbase]; an empty
-// support (kb > ke) gives the F = 2 branch.  Every lane of a group returns
-// its candidate's values.
-__device__ Eval eval_group(const float *src, int64_t sstride, int sbase, int kb, int ke,
-                           double t_data, double TA, double t0, double tau, int lane) {
-    const int gl = lane & (GROUP - 1);
-    const int gbase = lane & ~(GROUP - 1);
-    double v[16];
+// One exchange step for all sums at once (the moves of one step read values
+// written a step earlier, so they need no wait states between them).
+template <int CTRL, int N>
+__device__ __forceinline__ void group_step(double (&s)[N]) {
+    double t[N];
 #pragma unroll
-    for (int i = 0; i < 16; i++) v[i] = 0.0;
-    const double inv_tau = 1.0 / tau;
+    for (int i = 0; i < N; i++) t[i] = dpp_pair<CTRL>(s[i]);
+#pragma unroll
+    for (int i = 0; i < N; i++) s[i] += t[i];
+}
+
+// Sum each value over the 16 lanes of the group; every lane gets the same
+// totals.
+template <int N>
+__device__ __forceinline__ void group_sum16(double (&s)[N]) {
+    group_step<DPP_ROW_MIRROR>(s);
+    group_step<DPP_ROW_HALF_MIRROR>(s);
+    group_step<DPP_QUAD_XOR2>(s);
+    group_step<DPP_QUAD_XOR1>(s);
+}
+
+// Window sums of one candidate per 16-lane group (4 candidates per wave):
+// s[0..6] = S_c, s[7..13] = U_c.  Atom (c, k) of the template is
+// src[c*sstride + k - sbase] (src in LDS or in global memory); an empty
+// support (kb > ke) gives zeros.
+template <typename P>
+__device__ __forceinline__ void group_sums(P src, int64_t sstride, int sbase, int kb, int ke,
+                                           double t_data, double TA, double t0, double inv_tau,
+                                           int gl, double (&s)[2 * NC]) {
+#pragma unroll
+    for (int i = 0; i < 2 * NC; i++) s[i] = 0.0;
     const int k0 = kb + gl;
     double u = fma((double)k0, TA, t_data - t0);  // t_k - t0
     double w = exp(-u * inv_tau);
     const double rg = exp(-(double)GROUP * TA * inv_tau);
     const double du = (double)GROUP * TA;
     for (int k = k0; k <= ke; k += GROUP) {
-        const float *a = src + (k - sbase);
+        const int j = k - sbase;
         const double w2 = w * w;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const double x = (c < 3 ? w2 : w) * (double)a[c * sstride];
-            v[c] += x;
-            v[NC + c] = fma(u, x, v[NC + c]);
+            const double x = (c < 3 ? w2 : w) * (double)src[c * sstride + j];
+            s[c] += x;
+            s[NC + c] = fma(u, x, s[NC + c]);
         }
         w *= rg;
         u += du;
     }
-    const double red = group_transpose_sum16(v, gl);
-    double s[2 * NC];
-#pragma unroll
-    for (int j = 0; j < 2 * NC; j++) s[j] = __shfl(red, gbase + j, 64);
+    group_sum16(s);
+}
 
+// F and its derivatives from the window sums (group-uniform).
+__device__ __forceinline__ Eval eval_from_sums(const double (&s)[2 * NC], double inv_tau) {
     Eval e;
     const double A = s[0], B = s[1], C = s[2];
     const double far = s[3], fai = s[4], fbr = s[5], fbi = s[6];
@@ -156,24 +173,44 @@ __device__ Eval eval_group(const float *src, int64_t sstride, int sbase, int kb,
     return e;
 }
 
-// Stage the item's support union (component-major) into LDS when it fits and
-// return where atom (c, k) lives.
-__device__ __forceinline__ const float *stage_support(const RItem &it, const RTmpl &tp,
-                                                      const float *__restrict__ cols,
-                                                      int64_t cstride, float *lds,
-                                                      int64_t &sstride, int &sbase) {
-    if (it.in_lds) {
+// The atoms of an item: its LDS copy of the support union, or the template's
+// bins in global memory when the union does not fit.
+struct Src {
+    const float *g;  // global: template base
+    int64_t gstride;
+    int lstride, lbase;
+    bool in_lds;
+};
+
+__device__ __forceinline__ Eval eval_group(const Src &sr, const float *lds, int kb, int ke,
+                                           double t_data, double TA, double t0, double tau,
+                                           int gl) {
+    const double inv_tau = 1.0 / tau;
+    double s[2 * NC];
+    if (sr.in_lds)  // block-uniform
+        group_sums(lds, sr.lstride, sr.lbase, kb, ke, t_data, TA, t0, inv_tau, gl, s);
+    else
+        group_sums(sr.g, sr.gstride, 0, kb, ke, t_data, TA, t0, inv_tau, gl, s);
+    return eval_from_sums(s, inv_tau);
+}
+
+// Stage the item's support union (component-major) into LDS when it fits.
+__device__ __forceinline__ Src stage_support(const RItem &it, const RTmpl &tp,
+                                             const float *__restrict__ cols, int64_t cstride,
+                                             float *lds) {
+    Src sr;
+    sr.g = cols + tp.m_off;
+    sr.gstride = cstride;
+    sr.lstride = it.u_len;
+    sr.lbase = it.u_lo;
+    sr.in_lds = it.in_lds != 0;
+    if (sr.in_lds) {
         const float *g = cols + tp.m_off + it.u_lo;
         for (int c = 0; c < NC; c++)
             for (int k = threadIdx.x; k < it.u_len; k += RWG) lds[c * it.u_len + k] = g[c * cstride + k];
         __syncthreads();
-        sstride = it.u_len;
-        sbase = it.u_lo;
-        return lds;
     }
-    sstride = cstride;
-    sbase = 0;
-    return cols + tp.m_off;
+    return sr;
 }
 
 // Candidate slot of a lane: group (lane / 16) of wave w handles item
@@ -204,15 +241,12 @@ __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
     const RTmpl tp = tm[it.t];
-    int64_t sstride;
-    int sbase;
-    const float *src = stage_support(it, tp, cols, cstride, lds, sstride, sbase);
+    const Src sr = stage_support(it, tp, cols, cstride, lds);
     const int lane = threadIdx.x & 63, gl = lane & (GROUP - 1);
     const Slot sl = lane_slot(it, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
     const double tau = sl.has ? tauv[sl.ci] : 1.0;
-    const Eval e = eval_group(src, sstride, sbase, sl.kb, sl.ke, (double)tp.t_data, TA, t0, tau,
-                              lane);
+    const Eval e = eval_group(sr, lds, sl.kb, sl.ke, (double)tp.t_data, TA, t0, tau, gl);
     if (!sl.has) return;
     if (gl == 0) {
         twoF[sl.ci] = 2.0 * e.F;
@@ -238,9 +272,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
     const RTmpl tp = tm[it.t];
-    int64_t sstride;
-    int sbase;
-    const float *src = stage_support(it, tp, cols, cstride, lds, sstride, sbase);
+    const Src sr = stage_support(it, tp, cols, cstride, lds);
     const int lane = threadIdx.x & 63, gl = lane & (GROUP - 1);
     const Slot sl = lane_slot(it, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
@@ -250,7 +282,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     const double lo = (live && tau_lo) ? log(tau_lo[sl.ci]) : -INFINITY;
     const double hi = (live && tau_hi) ? log(tau_hi[sl.ci]) : INFINITY;
     const int kb = live ? sl.kb : 1, ke = live ? sl.ke : 0;
-    Eval e = eval_group(src, sstride, sbase, kb, ke, (double)tp.t_data, TA, t0, tau, lane);
+    Eval e = eval_group(sr, lds, kb, ke, (double)tp.t_data, TA, t0, tau, gl);
     double s = log(tau);
     double g = tau * e.dFdtau;  // dF/dlog(tau)
     double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
@@ -261,8 +293,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
         const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
         const bool moved = sn != s;
         const double taun = moved ? exp(sn) : tau;
-        const Eval en = eval_group(src, sstride, sbase, kb, ke, (double)tp.t_data, TA, t0, taun,
-                                   lane);
+        const Eval en = eval_group(sr, lds, kb, ke, (double)tp.t_data, TA, t0, taun, gl);
         const bool accept = moved && en.F > e.F;
         if (accept) {
             s = sn;
