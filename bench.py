@@ -56,6 +56,14 @@ WORKLOADS = {
 }
 
 REFINE_METRIC = "refined candidate-steps/s on 1 GPU (2F + d2F/dtau per step)"
+
+# HBM bytes per launch of the map kernel measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE (scaled by a 1 GiB copy in the same run) for the default-size
+# workloads; the commands and raw counters are in the named summaries.
+PROFILED_TRAFFIC = {
+    "rect_batched": (2.00e9 + 26.63e9, "profiles/rect_fast_kernel.rect_batched.md"),
+    "exp_shared": (0.124e9 + 0.545e9, "profiles/exp_fast_kernel.exp_shared.md"),
+}
 FP64_VALU_PEAK_TFLOPS = 78.6
 
 
@@ -517,9 +525,12 @@ def main():
     value = cells / (ms_per_step / 1e3)
     map_bytes = 4.0 * cells
     achieved = map_bytes / (stage_ms[1] / 1e3) / 1e9
-    traffic = None
+    traffic, traffic_src = None, None
     if args.traffic_kib:
         traffic = (args.traffic_kib[0] + args.traffic_kib[1]) * 1024.0
+        traffic_src = "--traffic-kib"
+    elif PROFILED_TRAFFIC.get(args.workload) and args.templates == 4096:
+        traffic, traffic_src = PROFILED_TRAFFIC[args.workload]
     job_bytes = 4.0 * cells + 28.0 * atoms_binned
     out = {
         "metric": METRIC,
@@ -553,6 +564,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": map_bytes,
             "launch_ms": stage_ms[1],
             "stage_ms": stage_ms[0],
