@@ -932,14 +932,14 @@ __device__ __forceinline__ ExpWalk expf_stage(const TmplDev &td, const double *X
     wk.nhead = td.a * (r_top - r_bot);
     wk.tail_lo = wk.head_lo + td.L0 + (int64_t)td.Ls * col0;
     const int ntail = wk.nhead + td.Ls * (col_last - col0);
-    wk.stride = wk.nhead + ntail;
+    wk.stride = wk.nhead + ntail;  // staged atoms; LDS planes are EXPF_LDS_CAP apart
     wk.in_lds = wk.stride <= lds_atoms;  // block-uniform
     if (wk.in_lds) {
         for (int idx = threadIdx.x; idx < wk.stride * NC; idx += WG) {
             const int j = idx / NC;
             const int c = idx - j * NC;
             const int64_t atom = j < wk.nhead ? wk.head_lo + j : wk.tail_lo + (j - wk.nhead);
-            lds[c * wk.stride + j] = atom < wk.H ? X[atom * MERGED_STRIDE + c] : 0.0;
+            lds[c * EXPF_LDS_CAP + j] = atom < wk.H ? X[atom * MERGED_STRIDE + c] : 0.0;
         }
     }
     __syncthreads();
@@ -958,8 +958,8 @@ __device__ __forceinline__ void expf_step(const ExpWalk &wk, const ExpCol &ec, i
             const int jt = wk.nhead + (int)(kt - wk.tail_lo);
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                xh[c] = wk.lds[c * wk.stride + jh];
-                xt[c] = wk.lds[c * wk.stride + jt];
+                xh[c] = wk.lds[c * EXPF_LDS_CAP + jh];
+                xt[c] = wk.lds[c * EXPF_LDS_CAP + jt];
             }
         } else {
             const double *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
@@ -986,8 +986,8 @@ __device__ __forceinline__ void expf_step1(const ExpWalk &wk, const ExpCol &ec, 
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int g = c < 3 ? 0 : 1;
-        const double xh = wk.lds[c * wk.stride + jh];
-        const double xt = wk.lds[c * wk.stride + jt];
+        const double xh = wk.lds[c * EXPF_LDS_CAP + jh];
+        const double xt = wk.lds[c * EXPF_LDS_CAP + jt];
         E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh));
     }
 }
@@ -1022,6 +1022,7 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
     if (wk.in_lds && td.a == 1) {  // block-uniform
         int jh = (int)(k - 1 - wk.head_lo);
         int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
+#pragma unroll 2
         for (int m = r_top; m > r_bot; m--) {
             expf_step1(wk, ec, jh, jt, E);
             jh--;
@@ -1067,6 +1068,48 @@ __global__ __launch_bounds__(WG) void exp_scan_kernel(const TmplDev *__restrict_
     for (int c = 0; c < NC; c++) expf_etab(etab, td, 0, c)[n] = E[c];
 }
 
+// Rows r_top .. row0 of one chunk walk in groups of 4 (RB is a multiple of 16),
+// reduced together.  Rows past the map (m > m_hi, only in a template's last
+// chunk) are evaluated but neither stored nor reduced, which keeps the loop
+// free of branches; only the chunk's final row takes no step.
+template <bool BTSG, bool STRICT, bool STEP1>
+__device__ __forceinline__ void expf_rows(const TmplDev &td, const ExpWalk &wk, const ExpCol &ec,
+                                          double (&E)[NC], Reducer<BTSG, 1> &red, bool &single,
+                                          float *__restrict__ Ft, int r_top, int row0, int m_hi,
+                                          int n, bool valid, int lane, float *sh_rowM,
+                                          float *sh_rowS) {
+    const int N_tau = (int)td.N_tau;
+    const int colidx[1] = {n};
+    int k = td.k00 + td.a * r_top;
+    int jh = (int)(k - 1 - wk.head_lo);
+    int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
+    for (int g = r_top; g >= row0; g -= 4) {
+        const bool last = g - 3 == row0;
+        float F4[4][1];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int m = g - j;
+            const float fv = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
+            const bool keep = valid && m <= m_hi;
+            if (keep) Ft[(int64_t)m * N_tau + n] = fv;
+            if (STRICT) single |= keep && (k >= (int)td.N - 1);
+            F4[3 - j][0] = keep ? fv : -INFINITY;  // F4[r] is row g - 3 + r
+            if (j < 3 || !last) {
+                if (STEP1) {
+                    expf_step1(wk, ec, jh, jt, E);
+                    jh--;
+                    jt--;
+                } else {
+                    expf_step(wk, ec, td.a, k, E);
+                }
+            }
+            k -= td.a;
+        }
+        if (g - 3 <= m_hi)
+            red.template add<4>(F4, g - 3, colidx, N_tau, lane, g - 3 - row0, sh_rowM, sh_rowS);
+    }
+}
+
 // The map: chunk rb walks rows (rb+1)*RB - 1 .. rb*RB from E_top(rb).
 template <bool BTSG, bool STRICT>
 __global__ __launch_bounds__(WG) void exp_fast_kernel(
@@ -1106,40 +1149,12 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     red.init();
     bool single = false;
     float *__restrict__ Ft = Fout + td.F_off;
-    const int colidx[1] = {n};
-    int k = td.k00 + td.a * r_top;
-    const bool step1 = wk.in_lds && td.a == 1;  // block-uniform
-    int jh = (int)(k - 1 - wk.head_lo);
-    int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
-    // rows in groups of 4 (RB is a multiple of 16), reduced together
-    for (int g = r_top; g >= row0; g -= 4) {
-        float F4[4][1];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int m = g - j;
-            float f = -INFINITY;
-            if (m <= m_hi) {  // wave-uniform: rows past the map are only walked
-                const float fv = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
-                if (valid) Ft[(int64_t)m * N_tau + n] = fv;
-                if (STRICT) single |= valid && (k >= (int)td.N - 1);
-                f = valid ? fv : -INFINITY;
-            }
-            F4[3 - j][0] = f;  // F4[r] is row g - 3 + r
-            if (m > row0) {
-                if (step1) {
-                    expf_step1(wk, ec, jh, jt, E);
-                    jh--;
-                    jt--;
-                } else {
-                    expf_step(wk, ec, td.a, k, E);
-                }
-            }
-            k -= td.a;
-        }
-        if (g - 3 <= m_hi)
-            red.template add<4>(F4, g - 3, colidx, N_tau, lane, g - 3 - row0, sh_rowM[wave],
-                                sh_rowS[wave]);
-    }
+    if (wk.in_lds && td.a == 1)  // block-uniform
+        expf_rows<BTSG, STRICT, true>(td, wk, ec, E, red, single, Ft, r_top, row0, m_hi, n, valid,
+                                      lane, sh_rowM[wave], sh_rowS[wave]);
+    else
+        expf_rows<BTSG, STRICT, false>(td, wk, ec, E, red, single, Ft, r_top, row0, m_hi, n, valid,
+                                       lane, sh_rowM[wave], sh_rowS[wave]);
     if (STRICT && single) atomicOr(flags + it.t, 1);
     // combine: each column belongs to one lane; each row to all 4 waves
     wave_argmax(red.best, red.bidx);
@@ -1608,8 +1623,8 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
             for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
                 p.col_items.push_back(ItemDev{(int32_t)t, 0, (int32_t)cbk, 0});
             const int need = expf_stage_atoms(td.a, rb + 1, td.Ls, TILE_COLS);
-            if (need <= EXPF_LDS_CAP)
-                p.max_lds_expf = std::max(p.max_lds_expf, (int)(sizeof(double) * NC * need));
+            if (need <= EXPF_LDS_CAP)  // planes at a fixed stride: the full capacity
+                p.max_lds_expf = (int)(sizeof(double) * NC * EXPF_LDS_CAP);
         }
     } else {
         td.prefix_off = p.prefix_doubles;
