@@ -73,8 +73,8 @@ struct TmplDev {
     int32_t q0, q00, a;          // MODE_RECT_FAST index maths; a also MODE_EXP_FAST
     int32_t k00, L0, Ls, H;      // MODE_EXP_FAST: first start atom, window lengths
                                  // L(n) = L0 + Ls*n atoms, atoms [0, H) reachable
-    int32_t n_seg, pad_seg;      // MODE_EXP_FAST: row chunks walked (>= n_rowblk)
-    int64_t etab_off;            // MODE_EXP_FAST: doubles, [n_seg][7][N_tau]
+    int32_t n_seg, pad_seg;      // MODE_EXP_FAST: row chunks (= n_rowblk)
+    int64_t etab_off;            // MODE_EXP_FAST: doubles, [n_seg + 1][7][N_tau]
     uint32_t o_t0, o_t0Band, o_tau, o_tauBand;  // caller's window (posterior modes)
 };
 
@@ -872,15 +872,17 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
 //     E(k - 1) = rho E(k) + x_{k-1} - rho^L x_{k-1+L}
 // (rho^2 and rho^2L = e^-6 for the a^2, b^2, ab columns; rho^L = e^-3), so a
 // cell costs O(1) instead of O(L).  This first-order linear recurrence runs
-// down the rows in chunks of RB rows, in parallel:
+// down the rows in chunks of RB rows (the map's row blocks), in parallel:
+//   exp_init_kernel  E at the top row of the top chunk, by the direct O(L) sum;
 //   exp_seg_kernel   Y(s): the recurrence across chunk s started from E = 0;
-//   exp_scan_kernel  per tau column, E at every chunk top from E = 0 at the
-//                    first atom no window reaches: E_top(s-1) = rho^(a RB)
-//                    E_top(s) + Y(s);
+//   exp_scan_kernel  per tau column, E at every chunk top:
+//                    E_top(s-1) = rho^(a RB) E_top(s) + Y(s);
 //   exp_fast_kernel  each chunk walks its rows from E_top, evaluating F.
 // Chunk s spans rows s*RB .. (s+1)*RB - 1 (rows past N_t0 are walked but not
-// output).  Atoms entering and leaving the windows of a chunk are staged in
-// LDS; one thread per tau column.
+// output).  Atoms at or past H (no output window reaches them) count as zero
+// everywhere, which keeps the recurrence exact for every output row.  Atoms
+// entering and leaving the windows of a chunk are staged in LDS; one thread
+// per tau column.
 
 struct ExpCol {
     double rho[2];   // rho^2 (a2, b2, ab), rho (Fa, Fb)
@@ -1040,6 +1042,37 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
     }
 }
 
+// E at the top row of the top chunk, r = n_seg*RB - 1, by the direct sum
+// E(k) = sum_{l < L} rho^l x_{k+l} (atoms at or past H count as zero, as in
+// the walks), into slot n_seg.  Work items (t, -, cb); one lane per tau column;
+// every lane of a wave reads the same atom, so the loads are broadcasts.
+__global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict__ tds,
+                                                      const ItemDev *__restrict__ items,
+                                                      const double *__restrict__ merged,
+                                                      double *__restrict__ etab) {
+    const ItemDev it = items[block_slot()];
+    const TmplDev td = tds[it.t];
+    const int n = it.cb * TILE_COLS + threadIdx.x;
+    if (n >= (int)td.N_tau) return;
+    const ExpCol ec = exp_col(td, n);
+    const int64_t k0 = (int64_t)td.k00 + (int64_t)td.a * ((int64_t)td.n_seg * td.rb_rows - 1);
+    const int64_t lr = min((int64_t)ec.L, (int64_t)td.H - k0);
+    const double *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
+    double E[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) E[c] = 0.0;
+    double w[2] = {1.0, 1.0};  // rho^(2l), rho^l
+#pragma unroll 2
+    for (int64_t l = 0; l < lr; l++) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) E[c] = fma(w[c < 3 ? 0 : 1], X[l * MERGED_STRIDE + c], E[c]);
+        w[0] *= ec.rho[0];
+        w[1] *= ec.rho[1];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) expf_etab(etab, td, td.n_seg, c)[n] = E[c];
+}
+
 // Per tau column: E at every chunk top.  Work items (t, -, cb).
 __global__ __launch_bounds__(WG) void exp_scan_kernel(const TmplDev *__restrict__ tds,
                                                       const ItemDev *__restrict__ items,
@@ -1054,7 +1087,7 @@ __global__ __launch_bounds__(WG) void exp_scan_kernel(const TmplDev *__restrict_
     const double mult[2] = {m1 * m1, m1};
     double E[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) E[c] = 0.0;
+    for (int c = 0; c < NC; c++) E[c] = expf_etab(etab, td, td.n_seg, c)[n];  // top chunk
     for (int s = td.n_seg - 1; s >= 1; s--) {
 #pragma unroll
         for (int c = 0; c < NC; c++) {
@@ -1609,14 +1642,12 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         td.merged_off = p.merged_doubles;
         p.merged_doubles += (int64_t)N * MERGED_STRIDE;
         if (mode == MODE_EXP_FAST) {
-            // chunks until the top row's window starts at or beyond H (E = 0 there)
-            const int64_t steps = ((int64_t)td.H - td.k00 + td.a - 1) / td.a;  // rows to reach H
-            int64_t nseg = (std::max<int64_t>(steps, 0) + 1 + rb - 1) / rb;
-            nseg = std::max<int64_t>(nseg, (int64_t)td.n_rowblk);
-            nseg = std::max<int64_t>(nseg, 1);
+            // chunks = the map's row blocks; E at the top chunk's top row comes
+            // from a direct sum (exp_init_kernel, slot n_seg of the table)
+            const int64_t nseg = std::max<int64_t>((int64_t)td.n_rowblk, 1);
             td.n_seg = (int32_t)nseg;
             td.etab_off = p.etab_doubles;
-            p.etab_doubles += nseg * NC * N_tau;
+            p.etab_doubles += (nseg + 1) * NC * N_tau;
             for (int64_t sg = 1; sg < nseg; sg++)
                 for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
                     p.seg_items.push_back(ItemDev{(int32_t)t, (int32_t)sg, (int32_t)cbk, 0});
@@ -1777,6 +1808,10 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(STAGE_WG), 0, st,
                        (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
                        cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
+    if (!p.col_items.empty())
+        hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)p.col_items.size()), dim3(WG), 0, st,
+                           (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
+                           (const double *)p.d_merged.p, (double *)p.d_etab.p);
     if (!p.seg_items.empty())
         hipLaunchKernelGGL(exp_seg_kernel, dim3((unsigned)p.seg_items.size()), dim3(WG),
                            p.max_lds_expf, st, (const TmplDev *)p.d_tmpl.p,
