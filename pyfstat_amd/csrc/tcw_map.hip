@@ -903,6 +903,9 @@ __device__ __forceinline__ ExpCol exp_col(const TmplDev &td, int n) {
 }
 
 constexpr int MAX_RB_EXPF = 128;
+// batches with fewer (template, column block) pairs than this walk the rows
+// above the map instead of summing the top chunk directly
+constexpr int64_t DIRECT_INIT_MIN_BLOCKS = 256;
 // atoms staged in LDS, 7 doubles each: 1088 * 56 B = 59.5 KB, which with the
 // kernel's 4 KB of row partials stays within a workgroup's 64 KB
 constexpr int EXPF_LDS_CAP = 1088;
@@ -1642,9 +1645,17 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         td.merged_off = p.merged_doubles;
         p.merged_doubles += (int64_t)N * MERGED_STRIDE;
         if (mode == MODE_EXP_FAST) {
-            // chunks = the map's row blocks; E at the top chunk's top row comes
-            // from a direct sum (exp_init_kernel, slot n_seg of the table)
-            const int64_t nseg = std::max<int64_t>((int64_t)td.n_rowblk, 1);
+            // E at the top chunk's top row comes from a direct O(L) sum
+            // (exp_init_kernel, slot n_seg of the table).  With enough
+            // independent column blocks in the batch the chunks are just the
+            // map's row blocks (least total work); otherwise extra chunks above
+            // the map reach H, where E = 0 and the direct sum is empty, so that
+            // no lane runs a long serial sum.
+            int64_t nseg = std::max<int64_t>((int64_t)td.n_rowblk, 1);
+            if (p.T * (int64_t)td.n_colblk < DIRECT_INIT_MIN_BLOCKS) {
+                const int64_t steps = ((int64_t)td.H - td.k00 + td.a - 1) / td.a;  // rows to H
+                nseg = std::max<int64_t>(nseg, (std::max<int64_t>(steps, 0) + rb) / rb);
+            }
             td.n_seg = (int32_t)nseg;
             td.etab_off = p.etab_doubles;
             p.etab_doubles += (nseg + 1) * NC * N_tau;
