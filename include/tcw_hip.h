@@ -266,7 +266,8 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
  *       candidates i whose support holds k of g_i dF_i/dS_c w_ik^{p_c}
  *   (p = 2 for a2, b2, ab; 1 for Fa, Fb), float32; every bin of every
  *   template is written (zero outside the supports); may be NULL.
- * saved is the forward's.  Asynchronous on `stream`. */
+ * saved is the forward's.  Asynchronous on `stream`; uses the plan's
+ * per-candidate scratch, so backward calls on one plan must not overlap. */
 int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau,
                         const double *saved, const double *g_twoF, const double *g_lnBtSG,
                         float *grad_cols, int64_t col_stride, double *grad_t0,

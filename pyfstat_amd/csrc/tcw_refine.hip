@@ -311,14 +311,45 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     }
 }
 
+// Per candidate, once per backward: g = 2 g_twoF + g_lnBtSG, the t0 / tau
+// gradients, and what the atom-gradient kernel needs: w_k = exp(-(x0 + k dx))
+// with x0 = (t_data - t0)/tau, dx = TAtom/tau, and coef_c = g dF/dS_c.
+struct CandCoef {
+    double x0, dx;
+    float coef[NC];
+    float pad;
+};
+static_assert(sizeof(CandCoef) == 48, "CandCoef layout");
+
+__global__ __launch_bounds__(256) void refine_coef_kernel(
+    int64_t n, const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t, double TA, const double *__restrict__ t0v,
+    const double *__restrict__ tauv, const double *__restrict__ saved,
+    const double *__restrict__ g2F, const double *__restrict__ glnB, double *__restrict__ gt0,
+    double *__restrict__ gtau, CandCoef *__restrict__ cc) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double gF = (g2F ? 2.0 * g2F[i] : 0.0) + (glnB ? glnB[i] : 0.0);
+    if (gt0) gt0[i] = gF * saved[i * NSAVED + 0];
+    if (gtau) gtau[i] = gF * saved[i * NSAVED + 1];
+    if (cc) {
+        const double inv = 1.0 / tauv[i];
+        CandCoef c;
+        c.x0 = ((double)tm[cand_t[i]].t_data - t0v[i]) * inv;
+        c.dx = TA * inv;
+#pragma unroll
+        for (int k = 0; k < NC; k++) c.coef[k] = (float)(gF * saved[i * NSAVED + 2 + k]);
+        c.pad = 0.0f;
+        cc[i] = c;
+    }
+}
+
 // grad_cols for bins b0 .. b0+nb-1 of one template: sum over the template's
-// candidates of g_i dF_i/dS_c w_ik^{p_c}.  Candidate data are uniform loads;
-// weights in FP32 (exp of an FP64 argument rounded once).
+// candidates of coef_c w_k^{p_c} (FP32; w from an FP64 argument rounded once).
+// Candidate data are uniform loads; candidates whose support misses the block
+// are skipped.
 __global__ __launch_bounds__(BWD_BINS) void refine_backward_kernel(
     const RTmpl *__restrict__ tm, const BItem *__restrict__ bitems, const int32_t *__restrict__ k_lo,
-    const int32_t *__restrict__ k_hi, double TA, const double *__restrict__ t0v,
-    const double *__restrict__ tauv, const double *__restrict__ saved,
-    const double *__restrict__ g2F, const double *__restrict__ glnB, float *__restrict__ grad,
+    const int32_t *__restrict__ k_hi, const CandCoef *__restrict__ cc, float *__restrict__ grad,
     int64_t cstride) {
     const BItem bi = bitems[blockIdx.x];
     const RTmpl tp = tm[bi.t];
@@ -330,35 +361,18 @@ __global__ __launch_bounds__(BWD_BINS) void refine_backward_kernel(
     for (int64_t i = tp.cand_beg; i < tp.cand_end; i++) {
         const int kb = k_lo[i], ke = k_hi[i];
         if (ke < blo || kb > bhi) continue;  // uniform
-        const double gF = (g2F ? 2.0 * g2F[i] : 0.0) + (glnB ? glnB[i] : 0.0);
-        if (gF == 0.0) continue;
-        const double tau = tauv[i];
-        const double x0 = (fma((double)blo, TA, (double)tp.t_data) - t0v[i]) / tau;
-        const double dx = TA / tau;
+        const CandCoef c = cc[i];
         if (k >= kb && k <= ke) {
-            const float w = expf((float)-fma((double)threadIdx.x, dx, x0));
+            const float w = expf((float)-fma((double)k, c.dx, c.x0));
             const float w2 = w * w;
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
-                const float coef = (float)(gF * saved[i * NSAVED + 2 + c]);
-                acc[c] = fmaf(coef, c < 3 ? w2 : w, acc[c]);
-            }
+            for (int q = 0; q < NC; q++) acc[q] = fmaf(c.coef[q], q < 3 ? w2 : w, acc[q]);
         }
     }
     if ((int)threadIdx.x < bi.nb) {
 #pragma unroll
         for (int c = 0; c < NC; c++) grad[c * cstride + tp.m_off + k] = acc[c];
     }
-}
-
-__global__ __launch_bounds__(256) void refine_param_grad_kernel(
-    int64_t n, const double *__restrict__ saved, const double *__restrict__ g2F,
-    const double *__restrict__ glnB, double *__restrict__ gt0, double *__restrict__ gtau) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double gF = (g2F ? 2.0 * g2F[i] : 0.0) + (glnB ? glnB[i] : 0.0);
-    if (gt0) gt0[i] = gF * saved[i * NSAVED + 0];
-    if (gtau) gtau[i] = gF * saved[i * NSAVED + 1];
 }
 
 }  // namespace
@@ -369,8 +383,10 @@ struct tcw_refine_plan {
     uint32_t TAtom = 0;
     int lds_bytes = 0;
     int64_t n_items = 0, n_bitems = 0;
-    DevBuf d_tmpl, d_items, d_bitems, d_klo, d_khi;
+    DevBuf d_tmpl, d_items, d_bitems, d_klo, d_khi, d_ctmpl, d_coef;
     ~tcw_refine_plan() {
+        d_ctmpl.release();
+        d_coef.release();
         d_tmpl.release();
         d_items.release();
         d_bitems.release();
@@ -392,7 +408,7 @@ extern "C" {
 
 int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_plan **out) {
     if (!h || !d || !out) return fail(TCW_ERR_INVALID, "tcw_refine_plan_create: null argument");
-    if (d->T <= 0) return fail(TCW_ERR_INVALID, "no templates");
+    if (d->T <= 0 || d->T > INT32_MAX) return fail(TCW_ERR_INVALID, "template count out of range");
     if (d->n_cand < 0 || d->n_cand > INT32_MAX)
         return fail(TCW_ERR_INVALID, "candidate count out of range");
     if (d->TAtom < 2) return fail(TCW_ERR_INVALID, "TAtom must be at least 2 seconds.");
@@ -456,7 +472,8 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     if ((rc = p->d_tmpl.ensure(tm.size() * sizeof(RTmpl))) ||
         (rc = p->d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(RItem))) ||
         (rc = p->d_bitems.ensure(bitems.size() * sizeof(BItem))) ||
-        (rc = p->d_klo.ensure(nc * sizeof(int32_t))) || (rc = p->d_khi.ensure(nc * sizeof(int32_t)))) {
+        (rc = p->d_klo.ensure(nc * sizeof(int32_t))) || (rc = p->d_khi.ensure(nc * sizeof(int32_t))) ||
+        (rc = p->d_ctmpl.ensure(nc * sizeof(int32_t))) || (rc = p->d_coef.ensure(nc * sizeof(CandCoef)))) {
         delete p;
         return rc;
     }
@@ -470,6 +487,12 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
         e = hipMemcpy(p->d_klo.p, d->k_lo, (size_t)d->n_cand * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && d->n_cand > 0)
         e = hipMemcpy(p->d_khi.p, d->k_hi, (size_t)d->n_cand * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && d->n_cand > 0) {
+        std::vector<int32_t> ct((size_t)d->n_cand);
+        for (int64_t t = 0; t < d->T; t++)
+            for (int64_t i = d->cand_off[t]; i < d->cand_off[t + 1]; i++) ct[(size_t)i] = (int32_t)t;
+        e = hipMemcpy(p->d_ctmpl.p, ct.data(), ct.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         delete p;
         return fail(TCW_ERR_DEVICE, std::string("refine plan upload: ") + hipGetErrorString(e));
@@ -507,17 +530,19 @@ int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau,
     if (rc) return rc;
     if (!t0 || !tau || !saved) return fail(TCW_ERR_INVALID, "tcw_refine_backward: null argument");
     hipStream_t st = (hipStream_t)stream;
-    if (grad_cols) {
-        if (col_stride < p->total_bins)
-            return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
+    if (grad_cols && col_stride < p->total_bins)
+        return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
+    if (p->n_cand > 0 && (grad_cols || grad_t0 || grad_tau))
+        hipLaunchKernelGGL(refine_coef_kernel, dim3((unsigned)((p->n_cand + 255) / 256)), dim3(256),
+                           0, st, p->n_cand, (const RTmpl *)p->d_tmpl.p,
+                           (const int32_t *)p->d_ctmpl.p, (double)p->TAtom, t0, tau, saved, g_twoF,
+                           g_lnBtSG, grad_t0, grad_tau,
+                           grad_cols ? (CandCoef *)p->d_coef.p : nullptr);
+    if (grad_cols)
         hipLaunchKernelGGL(refine_backward_kernel, dim3((unsigned)p->n_bitems), dim3(BWD_BINS), 0, st,
                            (const RTmpl *)p->d_tmpl.p, (const BItem *)p->d_bitems.p,
                            (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p,
-                           (double)p->TAtom, t0, tau, saved, g_twoF, g_lnBtSG, grad_cols, col_stride);
-    }
-    if ((grad_t0 || grad_tau) && p->n_cand > 0)
-        hipLaunchKernelGGL(refine_param_grad_kernel, dim3((unsigned)((p->n_cand + 255) / 256)),
-                           dim3(256), 0, st, p->n_cand, saved, g_twoF, g_lnBtSG, grad_t0, grad_tau);
+                           (const CandCoef *)p->d_coef.p, grad_cols, col_stride);
     TCW_HIP_TRY(hipGetLastError());
     return TCW_OK;
 }
