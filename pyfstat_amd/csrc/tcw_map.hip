@@ -74,7 +74,7 @@ struct TmplDev {
     int32_t k00, L0, Ls, H;      // MODE_EXP_FAST: first start atom, window lengths
                                  // L(n) = L0 + Ls*n atoms, atoms [0, H) reachable
     int32_t n_seg, pad_seg;      // MODE_EXP_FAST: row chunks (= n_rowblk)
-    int64_t etab_off;            // MODE_EXP_FAST: doubles, [n_seg + 1][7][N_tau]
+    int64_t etab_off;            // MODE_EXP_FAST: doubles, [n_seg + INIT_PARTS][7][N_tau]
     uint32_t o_t0, o_t0Band, o_tau, o_tauBand;  // caller's window (posterior modes)
 };
 
@@ -161,6 +161,7 @@ __global__ __launch_bounds__(STAGE_WG) void stage_kernel(
     double *__restrict__ prefix, double *__restrict__ merged) {
     const TmplDev td = tds[blockIdx.x];
     const bool want_prefix = td.mode == MODE_RECT_FAST || td.mode == MODE_RECT_GENERIC;
+    if (!want_prefix) return;  // exp templates: stage_exp_kernel (block-uniform)
     const uint32_t N = td.N, TA = td.TAtom;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -270,6 +271,43 @@ __global__ __launch_bounds__(STAGE_WG) void stage_kernel(
     }
     if (want_prefix && tid < NC)
         P[(int64_t)N * NC + tid] = df_value(df{sh_hi[tid][STAGE_WG / 64], sh_lo[tid][STAGE_WG / 64]});
+}
+
+// Exponential-window templates: the merged atoms as the FP64 table [N][8]
+// (7 columns + padding), one thread per bin, the same float32 detector-order
+// sums as stage_kernel.  Grid: (bin blocks, exp templates), y-strided.
+__global__ __launch_bounds__(256) void stage_exp_kernel(
+    const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
+    const SegDev *__restrict__ segs, const uint32_t *__restrict__ ts,
+    const float *__restrict__ cols, int64_t cstride, double *__restrict__ merged) {
+    for (int64_t e = blockIdx.y; e < n_exp; e += gridDim.y) {
+        const TmplDev td = tds[exp_tmpls[e]];
+        const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+        if (b >= td.N) continue;
+        float v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = 0.0f;
+        const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b * td.TAtom, t_hi = t_lo + td.TAtom;
+        for (int64_t X = td.seg_begin; X < td.seg_end; X++) {
+            const SegDev sg = segs[X];
+            int64_t lo = sg.begin, hi = sg.end;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((uint64_t)ts[mid] < t_lo)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            for (int64_t k = lo; k < sg.end && (uint64_t)ts[k] < t_hi; k++) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) v[c] += cols[c * cstride + k];
+            }
+        }
+        double *row = merged + td.merged_off + (int64_t)b * MERGED_STRIDE;
+#pragma unroll
+        for (int c = 0; c < NC; c++) row[c] = (double)v[c];
+        row[NC] = 0.0;
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -906,6 +944,7 @@ constexpr int MAX_RB_EXPF = 128;
 // batches with fewer (template, column block) pairs than this walk the rows
 // above the map instead of summing the top chunk directly
 constexpr int64_t DIRECT_INIT_MIN_BLOCKS = 256;
+constexpr int INIT_PARTS = 8;  // blocks sharing one column block's direct sum
 // atoms staged in LDS, 7 doubles each: 1088 * 56 B = 59.5 KB, which with the
 // kernel's 4 KB of row partials stays within a workgroup's 64 KB
 constexpr int EXPF_LDS_CAP = 1088;
@@ -1047,33 +1086,42 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
 
 // E at the top row of the top chunk, r = n_seg*RB - 1, by the direct sum
 // E(k) = sum_{l < L} rho^l x_{k+l} (atoms at or past H count as zero, as in
-// the walks), into slot n_seg.  Work items (t, -, cb); one lane per tau column;
-// every lane of a wave reads the same atom, so the loads are broadcasts.
+// the walks), split over INIT_PARTS blocks per (template, column block): part
+// p sums its share of l into slot n_seg + p; the scan adds the parts.  One
+// lane per tau column; every lane of a wave reads the same atom.
 __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict__ tds,
                                                       const ItemDev *__restrict__ items,
                                                       const double *__restrict__ merged,
                                                       double *__restrict__ etab) {
-    const ItemDev it = items[block_slot()];
+    const ItemDev it = items[blockIdx.x / INIT_PARTS];
+    const int part = (int)(blockIdx.x % INIT_PARTS);
     const TmplDev td = tds[it.t];
     const int n = it.cb * TILE_COLS + threadIdx.x;
     if (n >= (int)td.N_tau) return;
     const ExpCol ec = exp_col(td, n);
     const int64_t k0 = (int64_t)td.k00 + (int64_t)td.a * ((int64_t)td.n_seg * td.rb_rows - 1);
-    const int64_t lr = min((int64_t)ec.L, (int64_t)td.H - k0);
-    const double *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
+    const int64_t lr = max(min((int64_t)ec.L, (int64_t)td.H - k0), (int64_t)0);
+    const int64_t q = (lr + INIT_PARTS - 1) / INIT_PARTS;
+    const int64_t l0 = part * q, l1 = min(lr, l0 + q);
     double E[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) E[c] = 0.0;
-    double w[2] = {1.0, 1.0};  // rho^(2l), rho^l
+    if (l0 < l1) {
+        const double tau = (double)td.tau + (double)n * (double)td.dtau;
+        const double w1 = exp(-(double)l0 * (double)td.TAtom / tau);  // rho^l0
+        double w[2] = {w1 * w1, w1};
+        const double *__restrict__ X = merged + td.merged_off + (k0 + l0) * MERGED_STRIDE;
 #pragma unroll 2
-    for (int64_t l = 0; l < lr; l++) {
+        for (int64_t l = 0; l < l1 - l0; l++) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) E[c] = fma(w[c < 3 ? 0 : 1], X[l * MERGED_STRIDE + c], E[c]);
-        w[0] *= ec.rho[0];
-        w[1] *= ec.rho[1];
+            for (int c = 0; c < NC; c++)
+                E[c] = fma(w[c < 3 ? 0 : 1], X[l * MERGED_STRIDE + c], E[c]);
+            w[0] *= ec.rho[0];
+            w[1] *= ec.rho[1];
+        }
     }
 #pragma unroll
-    for (int c = 0; c < NC; c++) expf_etab(etab, td, td.n_seg, c)[n] = E[c];
+    for (int c = 0; c < NC; c++) expf_etab(etab, td, td.n_seg + part, c)[n] = E[c];
 }
 
 // Per tau column: E at every chunk top.  Work items (t, -, cb).
@@ -1090,7 +1138,10 @@ __global__ __launch_bounds__(WG) void exp_scan_kernel(const TmplDev *__restrict_
     const double mult[2] = {m1 * m1, m1};
     double E[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) E[c] = expf_etab(etab, td, td.n_seg, c)[n];  // top chunk
+    for (int c = 0; c < NC; c++) {  // top chunk: the direct sum's parts
+        E[c] = 0.0;
+        for (int pp = 0; pp < INIT_PARTS; pp++) E[c] += expf_etab(etab, td, td.n_seg + pp, c)[n];
+    }
     for (int s = td.n_seg - 1; s >= 1; s--) {
 #pragma unroll
         for (int c = 0; c < NC; c++) {
@@ -1486,6 +1537,8 @@ struct tcw_plan {
     std::vector<ItemDev> items;      // grouped by mode
     std::vector<ItemDev> seg_items;  // MODE_EXP_FAST (t, chunk, column block), chunk >= 1
     std::vector<ItemDev> col_items;  // MODE_EXP_FAST (t, -, column block)
+    std::vector<int32_t> exp_tmpls;  // templates needing the FP64 atom table
+    int64_t exp_max_N = 0;
     int64_t mode_begin[MODE_COUNT + 1] = {0, 0, 0, 0, 0};
     int64_t prefix_doubles = 0, merged_doubles = 0, rowpart_n = 0, colpart_n = 0;
     int64_t etab_doubles = 0;
@@ -1493,7 +1546,7 @@ struct tcw_plan {
     int max_lds_bytes = 0;  // rect_fast_kernel dynamic LDS
     int max_lds_expf = 0;   // exp_fast_kernel dynamic LDS
     DevBuf d_tmpl, d_segs, d_items, d_prefix, d_merged, d_ipart, d_rowpart, d_colpart,
-        d_flags, d_seg_items, d_col_items, d_etab;
+        d_flags, d_seg_items, d_col_items, d_etab, d_exp_tmpls;
     hipGraphExec_t graph = nullptr;
     hipStream_t cap_stream = nullptr;  // private stream the graph is captured on
     const void *g_ts = nullptr, *g_cols = nullptr, *g_F = nullptr, *g_stats = nullptr;
@@ -1504,6 +1557,8 @@ struct tcw_plan {
         items.clear();
         seg_items.clear();
         col_items.clear();
+        exp_tmpls.clear();
+        exp_max_N = 0;
         prefix_doubles = merged_doubles = rowpart_n = colpart_n = F_total = 0;
         etab_doubles = 0;
         max_lds_bytes = 0;
@@ -1526,6 +1581,7 @@ struct tcw_plan {
         d_seg_items.release();
         d_col_items.release();
         d_etab.release();
+        d_exp_tmpls.release();
     }
 };
 
@@ -1644,6 +1700,8 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
     if (mode == MODE_EXP_DIRECT || mode == MODE_EXP_FAST) {
         td.merged_off = p.merged_doubles;
         p.merged_doubles += (int64_t)N * MERGED_STRIDE;
+        p.exp_tmpls.push_back((int32_t)t);
+        p.exp_max_N = std::max<int64_t>(p.exp_max_N, N);
         if (mode == MODE_EXP_FAST) {
             // E at the top chunk's top row comes from a direct O(L) sum
             // (exp_init_kernel, slot n_seg of the table).  With enough
@@ -1658,7 +1716,7 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
             }
             td.n_seg = (int32_t)nseg;
             td.etab_off = p.etab_doubles;
-            p.etab_doubles += (nseg + 1) * NC * N_tau;
+            p.etab_doubles += (nseg + INIT_PARTS) * NC * N_tau;
             for (int64_t sg = 1; sg < nseg; sg++)
                 for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
                     p.seg_items.push_back(ItemDev{(int32_t)t, (int32_t)sg, (int32_t)cbk, 0});
@@ -1722,6 +1780,8 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     if ((rc = p.d_col_items.ensure(sizeof(ItemDev) * std::max<size_t>(p.col_items.size(), 1))))
         return rc;
     if ((rc = p.d_etab.ensure(sizeof(double) * std::max<int64_t>(p.etab_doubles, 1)))) return rc;
+    if ((rc = p.d_exp_tmpls.ensure(sizeof(int32_t) * std::max<size_t>(p.exp_tmpls.size(), 1))))
+        return rc;
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
         return async ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
@@ -1732,6 +1792,7 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     TCW_HIP_TRY(up(p.d_items.p, p.items.data(), sizeof(ItemDev) * p.items.size()));
     TCW_HIP_TRY(up(p.d_seg_items.p, p.seg_items.data(), sizeof(ItemDev) * p.seg_items.size()));
     TCW_HIP_TRY(up(p.d_col_items.p, p.col_items.data(), sizeof(ItemDev) * p.col_items.size()));
+    TCW_HIP_TRY(up(p.d_exp_tmpls.p, p.exp_tmpls.data(), sizeof(int32_t) * p.exp_tmpls.size()));
     return TCW_OK;
 }
 
@@ -1819,8 +1880,16 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(STAGE_WG), 0, st,
                        (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
                        cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
+    if (!p.exp_tmpls.empty())
+        hipLaunchKernelGGL(stage_exp_kernel,
+                           dim3((unsigned)((p.exp_max_N + 255) / 256),
+                                (unsigned)std::min<size_t>(p.exp_tmpls.size(), 65535)),
+                           dim3(256), 0, st, (const TmplDev *)p.d_tmpl.p,
+                           (const int32_t *)p.d_exp_tmpls.p, (int64_t)p.exp_tmpls.size(),
+                           (const SegDev *)p.d_segs.p, ts, cols, cstride, (double *)p.d_merged.p);
     if (!p.col_items.empty())
-        hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)p.col_items.size()), dim3(WG), 0, st,
+        hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)(p.col_items.size() * INIT_PARTS)),
+                           dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
                            (const double *)p.d_merged.p, (double *)p.d_etab.p);
     if (!p.seg_items.empty())
