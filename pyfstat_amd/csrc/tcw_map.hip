@@ -274,23 +274,26 @@ __global__ __launch_bounds__(STAGE_WG) void stage_kernel(
 }
 
 // Exponential-window templates: the merged atoms as the FP64 table [N][8]
-// (7 columns + padding), one thread per bin, the same float32 detector-order
-// sums as stage_kernel.  Grid: (bin blocks, exp templates), y-strided.
+// (7 columns + padding), the same float32 detector-order sums as
+// stage_kernel.  A thread owns STAGE_BINS consecutive bins: one binary search
+// per detector, then a linear walk.  Grid: (bin runs, exp templates), y-strided.
 __global__ __launch_bounds__(256) void stage_exp_kernel(
     const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ ts,
     const float *__restrict__ cols, int64_t cstride, double *__restrict__ merged) {
     for (int64_t e = blockIdx.y; e < n_exp; e += gridDim.y) {
         const TmplDev td = tds[exp_tmpls[e]];
-        const uint32_t b = blockIdx.x * 256u + threadIdx.x;
-        if (b >= td.N) continue;
-        float v[NC];
+        const uint32_t b0 = (blockIdx.x * 256u + threadIdx.x) * STAGE_BINS;
+        if (b0 >= td.N) continue;
+        float v[STAGE_BINS][NC];
 #pragma unroll
-        for (int c = 0; c < NC; c++) v[c] = 0.0f;
-        const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b * td.TAtom, t_hi = t_lo + td.TAtom;
+        for (int j = 0; j < STAGE_BINS; j++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) v[j][c] = 0.0f;
+        const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b0 * td.TAtom;
         for (int64_t X = td.seg_begin; X < td.seg_end; X++) {
             const SegDev sg = segs[X];
-            int64_t lo = sg.begin, hi = sg.end;
+            int64_t lo = sg.begin, hi = sg.end;  // first atom with ts >= t_lo
             while (lo < hi) {
                 const int64_t mid = (lo + hi) >> 1;
                 if ((uint64_t)ts[mid] < t_lo)
@@ -298,15 +301,25 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
                 else
                     hi = mid;
             }
-            for (int64_t k = lo; k < sg.end && (uint64_t)ts[k] < t_hi; k++) {
+            int64_t k = lo;
 #pragma unroll
-                for (int c = 0; c < NC; c++) v[c] += cols[c * cstride + k];
+            for (int j = 0; j < STAGE_BINS; j++) {
+                const uint64_t t_hi = t_lo + (uint64_t)(j + 1) * td.TAtom;
+                for (; k < sg.end && (uint64_t)ts[k] < t_hi; k++) {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) v[j][c] += cols[c * cstride + k];
+                }
             }
         }
-        double *row = merged + td.merged_off + (int64_t)b * MERGED_STRIDE;
 #pragma unroll
-        for (int c = 0; c < NC; c++) row[c] = (double)v[c];
-        row[NC] = 0.0;
+        for (int j = 0; j < STAGE_BINS; j++) {
+            if (b0 + j < td.N) {
+                double *row = merged + td.merged_off + (int64_t)(b0 + j) * MERGED_STRIDE;
+#pragma unroll
+                for (int c = 0; c < NC; c++) row[c] = (double)v[j][c];
+                row[NC] = 0.0;
+            }
+        }
     }
 }
 
@@ -1882,7 +1895,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
                        cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
     if (!p.exp_tmpls.empty())
         hipLaunchKernelGGL(stage_exp_kernel,
-                           dim3((unsigned)((p.exp_max_N + 255) / 256),
+                           dim3((unsigned)((p.exp_max_N + 256 * STAGE_BINS - 1) / (256 * STAGE_BINS)),
                                 (unsigned)std::min<size_t>(p.exp_tmpls.size(), 65535)),
                            dim3(256), 0, st, (const TmplDev *)p.d_tmpl.p,
                            (const int32_t *)p.d_exp_tmpls.p, (int64_t)p.exp_tmpls.size(),
