@@ -1101,7 +1101,9 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
 // E(k) = sum_{l < L} rho^l x_{k+l} (atoms at or past H count as zero, as in
 // the walks), split over INIT_PARTS blocks per (template, column block): part
 // p sums its share of l into slot n_seg + p; the scan adds the parts.  One
-// lane per tau column; every lane of a wave reads the same atom.
+// lane per tau column.  The l range is block-uniform (from the block's longest
+// window), so the atom loads are scalar; a lane's weight drops to zero past its
+// own window length.
 __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict__ tds,
                                                       const ItemDev *__restrict__ items,
                                                       const double *__restrict__ merged,
@@ -1109,23 +1111,29 @@ __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict_
     const ItemDev it = items[blockIdx.x / INIT_PARTS];
     const int part = (int)(blockIdx.x % INIT_PARTS);
     const TmplDev td = tds[it.t];
-    const int n = it.cb * TILE_COLS + threadIdx.x;
-    if (n >= (int)td.N_tau) return;
-    const ExpCol ec = exp_col(td, n);
+    const int col0 = it.cb * TILE_COLS;
+    const int col_last = min(col0 + TILE_COLS, (int)td.N_tau) - 1;
+    const int n = col0 + threadIdx.x;
+    const bool valid = n <= col_last;
+    const ExpCol ec = exp_col(td, valid ? n : col_last);
     const int64_t k0 = (int64_t)td.k00 + (int64_t)td.a * ((int64_t)td.n_seg * td.rb_rows - 1);
-    const int64_t lr = max(min((int64_t)ec.L, (int64_t)td.H - k0), (int64_t)0);
-    const int64_t q = (lr + INIT_PARTS - 1) / INIT_PARTS;
-    const int64_t l0 = part * q, l1 = min(lr, l0 + q);
+    const int64_t room = (int64_t)td.H - k0;
+    // longest window of the block (L grows with n), clipped at H: uniform
+    const int64_t lr_blk = max(min((int64_t)td.L0 + (int64_t)td.Ls * col_last, room), (int64_t)0);
+    const int64_t lr = max(min((int64_t)ec.L, room), (int64_t)0);  // this lane
+    const int64_t q = (lr_blk + INIT_PARTS - 1) / INIT_PARTS;
+    const int64_t l0 = part * q, l1 = min(lr_blk, l0 + q);
     double E[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) E[c] = 0.0;
     if (l0 < l1) {
-        const double tau = (double)td.tau + (double)n * (double)td.dtau;
+        const double tau = (double)td.tau + (double)(valid ? n : col_last) * (double)td.dtau;
         const double w1 = exp(-(double)l0 * (double)td.TAtom / tau);  // rho^l0
         double w[2] = {w1 * w1, w1};
-        const double *__restrict__ X = merged + td.merged_off + (k0 + l0) * MERGED_STRIDE;
-#pragma unroll 2
-        for (int64_t l = 0; l < l1 - l0; l++) {
+        const double *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
+#pragma unroll 4
+        for (int64_t l = l0; l < l1; l++) {
+            if (l == lr) w[0] = w[1] = 0.0;  // past this lane's window
 #pragma unroll
             for (int c = 0; c < NC; c++)
                 E[c] = fma(w[c < 3 ? 0 : 1], X[l * MERGED_STRIDE + c], E[c]);
@@ -1133,8 +1141,10 @@ __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict_
             w[1] *= ec.rho[1];
         }
     }
+    if (valid) {
 #pragma unroll
-    for (int c = 0; c < NC; c++) expf_etab(etab, td, td.n_seg + part, c)[n] = E[c];
+        for (int c = 0; c < NC; c++) expf_etab(etab, td, td.n_seg + part, c)[n] = E[c];
+    }
 }
 
 // Per tau column: E at every chunk top.  Work items (t, -, cb).
