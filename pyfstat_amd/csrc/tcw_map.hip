@@ -958,8 +958,9 @@ constexpr int MAX_RB_EXPF = 128;
 // above the map instead of summing the top chunk directly
 constexpr int64_t DIRECT_INIT_MIN_BLOCKS = 256;
 constexpr int INIT_PARTS = 8;  // blocks sharing one column block's direct sum
-// atoms staged in LDS, 7 doubles each: 1088 * 56 B = 59.5 KB, which with the
-// kernel's 4 KB of row partials stays within a workgroup's 64 KB
+// atoms staged in LDS as float32 (the merged atoms are float32 sums, so the
+// copy is exact), 7 planes of EXPF_LDS_CAP: 1088 * 28 B = 30.5 KB, so several
+// workgroups share a CU
 constexpr int EXPF_LDS_CAP = 1088;
 
 // Atoms a walk from row r_top down to row r_bot touches: entering atoms
@@ -971,14 +972,14 @@ __host__ __device__ __forceinline__ int expf_stage_atoms(int a, int rows, int Ls
 
 struct ExpWalk {
     const double *X;
-    double *lds;
+    float *lds;  // atom planes [c][EXPF_LDS_CAP]; merged atoms are float32 sums, so exact
     int64_t head_lo, tail_lo;
     int nhead, stride;  // LDS rows: [c][stride], head atoms then tail atoms
     int H;
     bool in_lds;
 };
 
-__device__ __forceinline__ ExpWalk expf_stage(const TmplDev &td, const double *X, double *lds,
+__device__ __forceinline__ ExpWalk expf_stage(const TmplDev &td, const double *X, float *lds,
                                               int lds_atoms, int r_top, int r_bot, int col0,
                                               int col_last) {
     ExpWalk wk;
@@ -996,7 +997,7 @@ __device__ __forceinline__ ExpWalk expf_stage(const TmplDev &td, const double *X
             const int j = idx / NC;
             const int c = idx - j * NC;
             const int64_t atom = j < wk.nhead ? wk.head_lo + j : wk.tail_lo + (j - wk.nhead);
-            lds[c * EXPF_LDS_CAP + j] = atom < wk.H ? X[atom * MERGED_STRIDE + c] : 0.0;
+            lds[c * EXPF_LDS_CAP + j] = atom < wk.H ? (float)X[atom * MERGED_STRIDE + c] : 0.0f;
         }
     }
     __syncthreads();
@@ -1015,8 +1016,8 @@ __device__ __forceinline__ void expf_step(const ExpWalk &wk, const ExpCol &ec, i
             const int jt = wk.nhead + (int)(kt - wk.tail_lo);
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                xh[c] = wk.lds[c * EXPF_LDS_CAP + jh];
-                xt[c] = wk.lds[c * EXPF_LDS_CAP + jt];
+                xh[c] = (double)wk.lds[c * EXPF_LDS_CAP + jh];
+                xt[c] = (double)wk.lds[c * EXPF_LDS_CAP + jt];
             }
         } else {
             const double *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
@@ -1043,8 +1044,8 @@ __device__ __forceinline__ void expf_step1(const ExpWalk &wk, const ExpCol &ec, 
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int g = c < 3 ? 0 : 1;
-        const double xh = wk.lds[c * EXPF_LDS_CAP + jh];
-        const double xt = wk.lds[c * EXPF_LDS_CAP + jt];
+        const double xh = (double)wk.lds[c * EXPF_LDS_CAP + jh];
+        const double xt = (double)wk.lds[c * EXPF_LDS_CAP + jt];
         E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh));
     }
 }
@@ -1059,7 +1060,7 @@ __device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, in
 __global__ __launch_bounds__(WG) void exp_seg_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items,
     const double *__restrict__ merged, double *__restrict__ etab, int lds_atoms) {
-    extern __shared__ double lds[];
+    extern __shared__ float lds[];  // staged atoms (exact float32 values)
     const ItemDev it = items[block_slot()];
     const TmplDev td = tds[it.t];
     const int RB = td.rb_rows;
@@ -1227,7 +1228,7 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     const double *__restrict__ merged, const double *__restrict__ etab,
     float *__restrict__ Fout, ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
     float *__restrict__ colpart, int *__restrict__ flags, int lds_atoms) {
-    extern __shared__ double lds[];
+    extern __shared__ float lds[];  // staged atoms (exact float32 values)
     __shared__ float sh_rowM[4][MAX_RB_EXPF];
     __shared__ float sh_rowS[4][MAX_RB_EXPF];
     __shared__ float sh_best[4];
@@ -1747,7 +1748,7 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
                 p.col_items.push_back(ItemDev{(int32_t)t, 0, (int32_t)cbk, 0});
             const int need = expf_stage_atoms(td.a, rb + 1, td.Ls, TILE_COLS);
             if (need <= EXPF_LDS_CAP)  // planes at a fixed stride: the full capacity
-                p.max_lds_expf = (int)(sizeof(double) * NC * EXPF_LDS_CAP);
+                p.max_lds_expf = (int)(sizeof(float) * NC * EXPF_LDS_CAP);
         }
     } else {
         td.prefix_off = p.prefix_doubles;
@@ -1880,7 +1881,7 @@ void launch_maps(tcw_plan &p, float *F, hipStream_t st) {
         hipLaunchKernelGGL((exp_fast_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG),
                            p.max_lds_expf, st, tds, items, p.mode_begin[MODE_EXP_FAST], merged,
                            (const double *)p.d_etab.p, F, ipart, rowpart, colpart, flags,
-                           p.max_lds_expf / (int)(sizeof(double) * NC));
+                           p.max_lds_expf / (int)(sizeof(float) * NC));
     if ((n = p.mode_begin[MODE_EXP_DIRECT + 1] - p.mode_begin[MODE_EXP_DIRECT]) > 0)
         hipLaunchKernelGGL((exp_direct_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG), 0,
                            st, tds, items, p.mode_begin[MODE_EXP_DIRECT], merged, F, ipart,
@@ -1919,7 +1920,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
         hipLaunchKernelGGL(exp_seg_kernel, dim3((unsigned)p.seg_items.size()), dim3(WG),
                            p.max_lds_expf, st, (const TmplDev *)p.d_tmpl.p,
                            (const ItemDev *)p.d_seg_items.p, (const double *)p.d_merged.p,
-                           (double *)p.d_etab.p, p.max_lds_expf / (int)(sizeof(double) * NC));
+                           (double *)p.d_etab.p, p.max_lds_expf / (int)(sizeof(float) * NC));
     if (!p.col_items.empty())
         hipLaunchKernelGGL(exp_scan_kernel, dim3((unsigned)p.col_items.size()), dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
