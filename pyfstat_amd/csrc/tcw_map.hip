@@ -1060,7 +1060,8 @@ __device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, in
 __global__ __launch_bounds__(WG) void exp_seg_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items,
     const double *__restrict__ merged, double *__restrict__ etab, int lds_atoms) {
-    extern __shared__ float lds[];  // staged atoms (exact float32 values)
+    extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
+    float *lds = reinterpret_cast<float *>(lds_raw);  // staged atoms, exact float32
     const ItemDev it = items[block_slot()];
     const TmplDev td = tds[it.t];
     const int RB = td.rb_rows;
@@ -1228,7 +1229,8 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     const double *__restrict__ merged, const double *__restrict__ etab,
     float *__restrict__ Fout, ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
     float *__restrict__ colpart, int *__restrict__ flags, int lds_atoms) {
-    extern __shared__ float lds[];  // staged atoms (exact float32 values)
+    extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
+    float *lds = reinterpret_cast<float *>(lds_raw);  // staged atoms, exact float32
     __shared__ float sh_rowM[4][MAX_RB_EXPF];
     __shared__ float sh_rowS[4][MAX_RB_EXPF];
     __shared__ float sh_best[4];
