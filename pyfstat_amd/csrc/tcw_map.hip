@@ -275,45 +275,88 @@ __global__ __launch_bounds__(STAGE_WG) void stage_kernel(
 
 // Exponential-window templates: the merged atoms as the FP64 table [N][8]
 // (7 columns + padding), the same float32 detector-order sums as
-// stage_kernel.  A thread owns STAGE_BINS consecutive bins: one binary search
-// per detector, then a linear walk.  Grid: (bin runs, exp templates), y-strided.
+// stage_kernel.  A workgroup owns SX_BINS_BLOCK consecutive bins of one
+// template: per detector, two binary searches bound the block's atoms, which
+// are copied into LDS in chunks with coalesced loads; each thread then sums its
+// SX_BINS bins from LDS (atoms in order, detectors in order).  Grid: (bin
+// blocks, exp templates), y-strided.
+constexpr int SX_BINS = 4;
+constexpr int SX_BINS_BLOCK = 256 * SX_BINS;
+constexpr int SX_CHUNK = 1024;  // atoms staged per pass
+
 __global__ __launch_bounds__(256) void stage_exp_kernel(
     const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ ts,
     const float *__restrict__ cols, int64_t cstride, double *__restrict__ merged) {
+    __shared__ uint32_t s_ts[SX_CHUNK];
+    __shared__ float s_x[NC][SX_CHUNK];
+    __shared__ int64_t s_rng[2];
+    const int tid = threadIdx.x;
     for (int64_t e = blockIdx.y; e < n_exp; e += gridDim.y) {
         const TmplDev td = tds[exp_tmpls[e]];
-        const uint32_t b0 = (blockIdx.x * 256u + threadIdx.x) * STAGE_BINS;
-        if (b0 >= td.N) continue;
-        float v[STAGE_BINS][NC];
+        const uint32_t B0 = blockIdx.x * (uint32_t)SX_BINS_BLOCK;
+        if (B0 >= td.N) continue;  // block-uniform
+        const uint32_t B1 = min(td.N, B0 + (uint32_t)SX_BINS_BLOCK);
+        const uint64_t TA = td.TAtom;
+        const uint64_t tb_lo = (uint64_t)td.t_min + (uint64_t)B0 * TA;
+        const uint64_t tb_hi = (uint64_t)td.t_min + (uint64_t)B1 * TA;
+        const uint32_t b0 = B0 + (uint32_t)tid * SX_BINS;  // this thread's first bin
+        float v[SX_BINS][NC];
 #pragma unroll
-        for (int j = 0; j < STAGE_BINS; j++)
+        for (int j = 0; j < SX_BINS; j++)
 #pragma unroll
             for (int c = 0; c < NC; c++) v[j][c] = 0.0f;
-        const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b0 * td.TAtom;
         for (int64_t X = td.seg_begin; X < td.seg_end; X++) {
             const SegDev sg = segs[X];
-            int64_t lo = sg.begin, hi = sg.end;  // first atom with ts >= t_lo
-            while (lo < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if ((uint64_t)ts[mid] < t_lo)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            int64_t k = lo;
-#pragma unroll
-            for (int j = 0; j < STAGE_BINS; j++) {
-                const uint64_t t_hi = t_lo + (uint64_t)(j + 1) * td.TAtom;
-                for (; k < sg.end && (uint64_t)ts[k] < t_hi; k++) {
-#pragma unroll
-                    for (int c = 0; c < NC; c++) v[j][c] += cols[c * cstride + k];
+            if (tid < 2) {  // first atom with ts >= tb_lo, resp. >= tb_hi
+                const uint64_t key = tid == 0 ? tb_lo : tb_hi;
+                int64_t lo = sg.begin, hi = sg.end;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if ((uint64_t)ts[mid] < key)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
                 }
+                s_rng[tid] = lo;
+            }
+            __syncthreads();
+            const int64_t ka = s_rng[0], kb = s_rng[1];
+            __syncthreads();  // s_rng is rewritten by the next detector
+            for (int64_t c0 = ka; c0 < kb; c0 += SX_CHUNK) {
+                const int na = (int)min((int64_t)SX_CHUNK, kb - c0);
+                for (int i = tid; i < na; i += 256) {
+                    s_ts[i] = ts[c0 + i];
+#pragma unroll
+                    for (int c = 0; c < NC; c++) s_x[c][i] = cols[c * cstride + c0 + i];
+                }
+                __syncthreads();
+                if (b0 < B1) {
+                    const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b0 * TA;
+                    int lo = 0, hi = na;  // first staged atom with ts >= t_lo
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if ((uint64_t)s_ts[mid] < t_lo)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    int k = lo;
+#pragma unroll
+                    for (int j = 0; j < SX_BINS; j++) {
+                        const uint64_t t_hi = t_lo + (uint64_t)(j + 1) * TA;
+                        for (; k < na && (uint64_t)s_ts[k] < t_hi; k++) {
+#pragma unroll
+                            for (int c = 0; c < NC; c++) v[j][c] += s_x[c][k];
+                        }
+                    }
+                }
+                __syncthreads();
             }
         }
 #pragma unroll
-        for (int j = 0; j < STAGE_BINS; j++) {
-            if (b0 + j < td.N) {
+        for (int j = 0; j < SX_BINS; j++) {
+            if (b0 + j < B1) {
                 double *row = merged + td.merged_off + (int64_t)(b0 + j) * MERGED_STRIDE;
 #pragma unroll
                 for (int c = 0; c < NC; c++) row[c] = (double)v[j][c];
@@ -1908,7 +1951,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
                        cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
     if (!p.exp_tmpls.empty())
         hipLaunchKernelGGL(stage_exp_kernel,
-                           dim3((unsigned)((p.exp_max_N + 256 * STAGE_BINS - 1) / (256 * STAGE_BINS)),
+                           dim3((unsigned)((p.exp_max_N + SX_BINS_BLOCK - 1) / SX_BINS_BLOCK),
                                 (unsigned)std::min<size_t>(p.exp_tmpls.size(), 65535)),
                            dim3(256), 0, st, (const TmplDev *)p.d_tmpl.p,
                            (const int32_t *)p.d_exp_tmpls.p, (int64_t)p.exp_tmpls.size(),
