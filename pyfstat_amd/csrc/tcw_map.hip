@@ -62,8 +62,8 @@ struct TmplDev {
     int64_t prefix_off;          // doubles, prefix entries [N+1][7]  (rect modes)
     int64_t F_off;               // floats, map [N_t0][N_tau]
     int64_t item_begin;          // first work item of this template
-    int64_t rowpart_off;         // float, [N_t0][n_colblk], relative to the item max
-    int64_t colpart_off;         // float, [N_tau][n_rowblk], relative to the item max
+    int64_t rowpart_off;         // float, [n_colblk][N_t0], relative to the item max
+    int64_t colpart_off;         // float, [n_rowblk][N_tau], relative to the item max
     uint32_t N, TAtom, t_min;    // binned atoms, atom length, t0_data
     uint32_t type, t0, dt0, tau, dtau;  // effective window (after NONE -> RECT)
     uint32_t N_t0, N_tau;
@@ -601,11 +601,11 @@ __device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const
 #pragma unroll
             for (int w = 0; w < 4; w++)
                 if (sh_M[w] != -INFINITY) S += sh_col[w][tid] * exp_rel(sh_M[w], M);
-            colpart[td.colpart_off + (int64_t)n * td.n_rowblk + it.rb] = S;
+            colpart[td.colpart_off + (int64_t)it.rb * td.N_tau + n] = S;
         }
         for (int rl = tid; rl < nrows_item; rl += WG) {
             const int m = it.rb * td.rb_rows + rl;
-            rowpart[td.rowpart_off + (int64_t)m * td.n_colblk + it.cb] =
+            rowpart[td.rowpart_off + (int64_t)it.cb * td.N_t0 + m] =
                 sh_rowS[rl] * exp_rel(sh_rowM[rl], M);
         }
     }
@@ -1326,7 +1326,7 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     const float M = fmaxf(fmaxf(sh_M[0], sh_M[1]), fmaxf(sh_M[2], sh_M[3]));
     if (BTSG) {
         if (valid)
-            colpart[td.colpart_off + (int64_t)n * td.n_rowblk + it.rb] =
+            colpart[td.colpart_off + (int64_t)it.rb * td.N_tau + n] =
                 red.Mw == -INFINITY ? 0.0f : red.Scol[0] * exp_rel(red.Mw, M);
         const int nrows = m_hi - row0 + 1;
         for (int rl = threadIdx.x; rl < nrows; rl += WG) {
@@ -1334,7 +1334,7 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
 #pragma unroll
             for (int w = 0; w < 4; w++)
                 if (sh_M[w] != -INFINITY) S += sh_rowS[w][rl] * exp_rel(sh_rowM[w][rl], M);
-            rowpart[td.rowpart_off + (int64_t)(row0 + rl) * td.n_colblk + it.cb] = S;
+            rowpart[td.rowpart_off + (int64_t)it.cb * td.N_t0 + row0 + rl] = S;
         }
     }
     if (threadIdx.x == 0) {
@@ -1453,10 +1453,10 @@ __global__ __launch_bounds__(WG) void finalize_kernel(
         int ri = 0x7fffffff;
         for (int m = threadIdx.x; m < (int)td.N_t0; m += WG) {
             const int rb = m / td.rb_rows;
-            const float *rp = rowpart + td.rowpart_off + (int64_t)m * td.n_colblk;
+            const float *rp = rowpart + td.rowpart_off + m;  // [colblk][N_t0]
             double R = 0.0;
             for (uint32_t cb = 0; cb < td.n_colblk; cb++)
-                R += (double)(rp[cb] * scale((int64_t)rb * td.n_colblk + cb));
+                R += (double)(rp[(int64_t)cb * td.N_t0] * scale((int64_t)rb * td.n_colblk + cb));
             if (R > rv || (R == rv && m < ri)) {
                 rv = R;
                 ri = m;
@@ -1468,10 +1468,10 @@ __global__ __launch_bounds__(WG) void finalize_kernel(
         int ci = 0x7fffffff;
         for (int n = threadIdx.x; n < (int)td.N_tau; n += WG) {
             const int cb = n / TILE_COLS;
-            const float *cp = colpart + td.colpart_off + (int64_t)n * td.n_rowblk;
+            const float *cp = colpart + td.colpart_off + n;  // [rowblk][N_tau]
             double Cs = 0.0;
             for (uint32_t rb = 0; rb < td.n_rowblk; rb++)
-                Cs += (double)(cp[rb] * scale((int64_t)rb * td.n_colblk + cb));
+                Cs += (double)(cp[(int64_t)rb * td.N_tau] * scale((int64_t)rb * td.n_colblk + cb));
             if (Cs > cv || (Cs == cv && n < ci)) {
                 cv = Cs;
                 ci = n;
