@@ -1356,11 +1356,10 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
 // finalize: per-template stats from the partials
 // ----------------------------------------------------------------------------
 
-constexpr int FIN_WG = 1024;  // finalize: one workgroup per template
-
+template <int NT>
 __device__ void block_argmax(float &bv, int &bi) {
-    __shared__ float sv[FIN_WG / 64];
-    __shared__ int si[FIN_WG / 64];
+    __shared__ float sv[NT / 64];
+    __shared__ int si[NT / 64];
     wave_argmax(bv, bi);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
@@ -1370,27 +1369,29 @@ __device__ void block_argmax(float &bv, int &bi) {
     __syncthreads();
     bv = sv[0];
     bi = si[0];
-    for (int w = 1; w < FIN_WG / 64; w++) argmax_merge(bv, bi, sv[w], si[w]);
+    for (int w = 1; w < NT / 64; w++) argmax_merge(bv, bi, sv[w], si[w]);
     __syncthreads();
 }
 
+template <int NT>
 __device__ double block_sum_d(double v) {
-    __shared__ double sd[FIN_WG / 64];
+    __shared__ double sd[NT / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) sd[wave] = v;
     __syncthreads();
     double s = 0.0;
-    for (int w = 0; w < FIN_WG / 64; w++) s += sd[w];
+    for (int w = 0; w < NT / 64; w++) s += sd[w];
     __syncthreads();
     return s;
 }
 
 // argmax over doubles with first-index tie break, block-wide
+template <int NT>
 __device__ void block_argmax_d(double &bv, int &bi) {
-    __shared__ double sv[FIN_WG / 64];
-    __shared__ int si[FIN_WG / 64];
+    __shared__ double sv[NT / 64];
+    __shared__ int si[NT / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         double ov = __shfl_xor(bv, o);
@@ -1408,7 +1409,7 @@ __device__ void block_argmax_d(double &bv, int &bi) {
     __syncthreads();
     bv = sv[0];
     bi = si[0];
-    for (int w = 1; w < FIN_WG / 64; w++)
+    for (int w = 1; w < NT / 64; w++)
         if (sv[w] > bv || (sv[w] == bv && si[w] < bi)) {
             bv = sv[w];
             bi = si[w];
@@ -1416,10 +1417,13 @@ __device__ void block_argmax_d(double &bv, int &bi) {
     __syncthreads();
 }
 
-constexpr int FIN_SCALE_CAP = 4096;  // per-item scale factors kept in LDS
+constexpr int FIN_SCALE_CAP = 4096;
+constexpr int64_t FIN_WIDE_MAX_T = 256;  // batches smaller than this finalize with 1024 threads  // per-item scale factors kept in LDS
 
-template <bool BTSG>
-__global__ __launch_bounds__(FIN_WG) void finalize_kernel(
+// One workgroup of NT threads per template (NT = 1024 for small batches, whose
+// maps are large, 256 otherwise).
+template <bool BTSG, int NT>
+__global__ __launch_bounds__(NT) void finalize_kernel(
     const TmplDev *__restrict__ tds, const ItemPart *__restrict__ ipart,
     const float *__restrict__ rowpart, const float *__restrict__ colpart,
     const int *__restrict__ flags, tcw_stats *__restrict__ stats) {
@@ -1430,8 +1434,8 @@ __global__ __launch_bounds__(FIN_WG) void finalize_kernel(
     const ItemPart *ip = ipart + td.item_begin;
     float bv = -INFINITY;
     int bi = 0x7fffffff;
-    for (int64_t k = threadIdx.x; k < nitems; k += FIN_WG) argmax_merge(bv, bi, ip[k].maxF, ip[k].idx);
-    block_argmax(bv, bi);
+    for (int64_t k = threadIdx.x; k < nitems; k += NT) argmax_merge(bv, bi, ip[k].maxF, ip[k].idx);
+    block_argmax<NT>(bv, bi);
     const float maxF = bv;
     const int64_t m_ML = bi / (int)td.N_tau;
     const int64_t n_ML = bi - m_ML * (int64_t)td.N_tau;
@@ -1441,19 +1445,19 @@ __global__ __launch_bounds__(FIN_WG) void finalize_kernel(
     if (BTSG) {
         // item k's partials are relative to its maximum M_k <= maxF
         double s = 0.0;
-        for (int64_t k = threadIdx.x; k < nitems; k += FIN_WG) {
+        for (int64_t k = threadIdx.x; k < nitems; k += NT) {
             const float sc = exp_rel(ip[k].M, maxF);
             if (k < FIN_SCALE_CAP) sh_scale[k] = sc;
             s += (double)ip[k].S * exp((double)ip[k].M - (double)maxF);
         }
-        s = block_sum_d(s);  // contains a barrier: sh_scale is complete
+        s = block_sum_d<NT>(s);  // contains a barrier: sh_scale is complete
         lnB = log(70.0 / ((double)td.N_t0 * (double)td.N_tau)) + (double)maxF + log(s);
         auto scale = [&](int64_t k) -> float {
             return k < FIN_SCALE_CAP ? sh_scale[k] : exp_rel(ip[k].M, maxF);
         };
         double rv = -1.0;
         int ri = 0x7fffffff;
-        for (int m = threadIdx.x; m < (int)td.N_t0; m += FIN_WG) {
+        for (int m = threadIdx.x; m < (int)td.N_t0; m += NT) {
             const int rb = m / td.rb_rows;
             const float *rp = rowpart + td.rowpart_off + m;  // [colblk][N_t0]
             double R = 0.0;
@@ -1464,11 +1468,11 @@ __global__ __launch_bounds__(FIN_WG) void finalize_kernel(
                 ri = m;
             }
         }
-        block_argmax_d(rv, ri);
+        block_argmax_d<NT>(rv, ri);
         m_MP = ri;
         double cv = -1.0;
         int ci = 0x7fffffff;
-        for (int n = threadIdx.x; n < (int)td.N_tau; n += FIN_WG) {
+        for (int n = threadIdx.x; n < (int)td.N_tau; n += NT) {
             const int cb = n / TILE_COLS;
             const float *cp = colpart + td.colpart_off + n;  // [rowblk][N_tau]
             double Cs = 0.0;
@@ -1479,7 +1483,7 @@ __global__ __launch_bounds__(FIN_WG) void finalize_kernel(
                 ci = n;
             }
         }
-        block_argmax_d(cv, ci);
+        block_argmax_d<NT>(cv, ci);
         n_MP = ci;
         t0_MP = (double)td.o_t0 + ((double)m_MP + 0.5) * ((double)td.o_t0Band / (double)td.N_t0);
         tau_MP = (double)td.o_tau + ((double)n_MP + 0.5) * ((double)td.o_tauBand / (double)td.N_tau);
@@ -1979,7 +1983,13 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
         else
             launch_maps<true, false>(p, F, st);
         if (ev) TCW_HIP_TRY(hipEventRecord(ev[2], st));
-        hipLaunchKernelGGL(finalize_kernel<true>, dim3((unsigned)p.T), dim3(FIN_WG), 0, st,
+        if (p.T < FIN_WIDE_MAX_T)
+            hipLaunchKernelGGL((finalize_kernel<true, 1024>), dim3((unsigned)p.T), dim3(1024), 0, st,
+                           (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
+                           (const float *)p.d_rowpart.p, (const float *)p.d_colpart.p,
+                           (const int *)p.d_flags.p, stats_dev);
+        else
+            hipLaunchKernelGGL((finalize_kernel<true, WG>), dim3((unsigned)p.T), dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
                            (const float *)p.d_rowpart.p, (const float *)p.d_colpart.p,
                            (const int *)p.d_flags.p, stats_dev);
@@ -1989,7 +1999,13 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
         else
             launch_maps<false, false>(p, F, st);
         if (ev) TCW_HIP_TRY(hipEventRecord(ev[2], st));
-        hipLaunchKernelGGL(finalize_kernel<false>, dim3((unsigned)p.T), dim3(FIN_WG), 0, st,
+        if (p.T < FIN_WIDE_MAX_T)
+            hipLaunchKernelGGL((finalize_kernel<false, 1024>), dim3((unsigned)p.T), dim3(1024), 0, st,
+                           (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
+                           (const float *)p.d_rowpart.p, (const float *)p.d_colpart.p,
+                           (const int *)p.d_flags.p, stats_dev);
+        else
+            hipLaunchKernelGGL((finalize_kernel<false, WG>), dim3((unsigned)p.T), dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemPart *)p.d_ipart.p,
                            (const float *)p.d_rowpart.p, (const float *)p.d_colpart.p,
                            (const int *)p.d_flags.p, stats_dev);
