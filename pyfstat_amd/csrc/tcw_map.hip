@@ -382,7 +382,9 @@ __device__ __forceinline__ float cell_F(double A, double B, double C, double far
     const double q = fma(fbr, fbr, fbi * fbi);
     const double r = fma(far, fbr, fai * fbi);
     const double num = fma(B, p, fma(A, q, (-2.0 * C) * r));
-    const bool ok = (T > 0.0) && (T * T < QCOND * D);
+    // A, B >= 0 (sums of squares), so T <= 0 only for an empty window, where
+    // D = 0 and the test below fails as well
+    const bool ok = T * T < QCOND * D;
     const float F = (float)(num * __builtin_amdgcn_rcp(D));
     return ok ? F : 2.0f;
 }
@@ -413,16 +415,56 @@ __device__ __forceinline__ float exp_rel(float x, float ref) {
     return __builtin_amdgcn_exp2f((x - ref) * LOG2E);  // exp(x - ref); exp(-inf) = 0
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
+
+// Cross-lane moves without LDS: DPP within 16-lane rows (quad permutes and
+// row rotations; every source lane lies inside the row, so all lanes are
+// written) and gfx950's row swaps across rows.
+constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_ROR4 = 0x124;  // row_ror:4
+constexpr int DPP_ROW_ROR8 = 0x128;  // row_ror:8
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Combine over the four 16-lane rows at each row position: v_permlane16_swap
+// pairs rows (0,1) and (2,3), v_permlane32_swap the two halves; every lane of
+// a position ends with the same bits.
+template <typename Op>
+__device__ __forceinline__ float rows16_reduce(float v, Op op) {
+    const unsigned u = __float_as_uint(v);
+    const auto p = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    const float s = op(__uint_as_float(p[0]), __uint_as_float(p[1]));
+    const unsigned us = __float_as_uint(s);
+    const auto q = __builtin_amdgcn_permlane32_swap(us, us, false, false);
+    return op(__uint_as_float(q[0]), __uint_as_float(q[1]));
+}
+
+__device__ __forceinline__ float rows16_sum(float v) {
+    return rows16_reduce(v, [](float a, float b) { return a + b; });
+}
+
+// Sum over the 4 quads of a 16-lane row at each quad position.
+__device__ __forceinline__ float quads_sum(float v) {
+    v += dpp_f<DPP_ROW_ROR4>(v);
+    return v + dpp_f<DPP_ROW_ROR8>(v);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_f<DPP_QUAD_XOR1>(v);
+    v += dpp_f<DPP_QUAD_XOR2>(v);
+    return rows16_sum(quads_sum(v));
+}
+
+// wave maximum, the same value on every lane
+__device__ __forceinline__ float wave_max(float v) {
+    v = fmaxf(v, dpp_f<DPP_QUAD_XOR1>(v));
+    v = fmaxf(v, dpp_f<DPP_QUAD_XOR2>(v));
+    v = fmaxf(v, dpp_f<DPP_ROW_ROR4>(v));
+    v = fmaxf(v, dpp_f<DPP_ROW_ROR8>(v));
+    return rows16_reduce(v, [](float a, float b) { return fmaxf(a, b); });
 }
 
 __device__ __forceinline__ void argmax_merge(float &bv, int &bi, float v, int i) {
@@ -445,10 +487,11 @@ constexpr int MAX_RB = 512;
 constexpr float REF_SLACK = 60.0f;  // e^60 * cells per item stays far below FLT_MAX
 
 // Wave sums of R per-lane values.  Returns the sum of one of them, and in
-// `row` which one (-1: this lane holds none).  R = 1: a plain butterfly, lane
-// 0 holds the result.  R = 4: two halving exchange steps first (lanes pair up
-// and keep half of the rows each), then a butterfly on one value per lane;
-// row r ends up on lane 2*(r&1) + (r>>1) -- 7 shuffles instead of 24.
+// `row` which one (-1: this lane holds none).  R = 1: lane 0 holds the sum.
+// R = 4: two halving exchange steps within each quad first (lanes pair up and
+// keep half of the rows each), then one value per lane is summed over the
+// quads (row rotations) and the rows (row swaps); row r ends up on lane
+// 2*(r&1) + (r>>1).
 template <int R>
 __device__ __forceinline__ float rows_sum(const float (&v)[R], int lane, int &row);
 
@@ -464,18 +507,13 @@ __device__ __forceinline__ float rows_sum<4>(const float (&v)[4], int lane, int 
     // step 1: b = 0 keeps rows {0,1}, b = 1 keeps rows {2,3}
     const float s0 = b ? v[0] : v[2];
     const float s1 = b ? v[1] : v[3];
-    const float r0 = __shfl_xor(s0, 1);
-    const float r1 = __shfl_xor(s1, 1);
-    const float u0 = (b ? v[2] : v[0]) + r0;
-    const float u1 = (b ? v[3] : v[1]) + r1;
+    const float u0 = (b ? v[2] : v[0]) + dpp_f<DPP_QUAD_XOR1>(s0);
+    const float u1 = (b ? v[3] : v[1]) + dpp_f<DPP_QUAD_XOR1>(s1);
     // step 2: c = 0 keeps the first of the pair, c = 1 the second
     const bool c = lane & 2;
-    const float z = (c ? u1 : u0) + __shfl_xor(c ? u0 : u1, 2);
-    float t = z;
-#pragma unroll
-    for (int o = 4; o < 64; o <<= 1) t += __shfl_xor(t, o);
+    const float z = (c ? u1 : u0) + dpp_f<DPP_QUAD_XOR2>(c ? u0 : u1);
     row = lane < 4 ? 2 * (b ? 1 : 0) + (c ? 1 : 0) : -1;
-    return t;
+    return rows16_sum(quads_sum(z));
 }
 
 // Per-work-item reduction state.  A lane owns 4 columns; every sub-tile brings
@@ -683,12 +721,13 @@ void rect_fast_kernel(
     const double *__restrict__ P = prefix + td.prefix_off;
     const int Nm1 = (int)td.N - 1;
 
-    for (int idx = threadIdx.x; idx < nd * NC; idx += WG) {
-        const int d = idx / NC;
-        const int c = idx - d * NC;
+    for (int d = threadIdx.x; d < nd; d += WG) {
         int64_t q = (int64_t)td.q00 + (int64_t)td.a * (D0 + d) - 1;
         q = min(max(q, (int64_t)0), (int64_t)Nm1);
-        lds[c * ndp + lds_pad(d)] = P[(q + 1) * NC + c];
+        const double *__restrict__ src = P + (q + 1) * NC;
+        double *dst = lds + lds_pad(d);
+#pragma unroll
+        for (int c = 0; c < NC; c++) dst[c * ndp] = src[c];
     }
     __syncthreads();
 
