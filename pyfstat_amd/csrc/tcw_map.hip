@@ -94,183 +94,187 @@ struct ItemDev {
 struct ItemPart {
     float maxF;
     int32_t idx;
-    float M, S;  // M = the item's max F; S = sum of exp(F - M) over its cells
+    float M, S;  // M = the item's max F times LOG2E (float); S = sum of exp(F - M/LOG2E)
 };
 
 // ----------------------------------------------------------------------------
-// FP32 error-free transformations (two-sum) for the compensated prefix scan
+// stage: binned merge + FP64 prefix (rect) / FP64 atom table (exp)
 // ----------------------------------------------------------------------------
 
-struct df {
-    float hi, lo;
-};
+constexpr int SR_WG = 256;                        // 4 wave64
+constexpr int SR_BINS = 4;                        // consecutive bins per thread per round
+constexpr int SR_ROUND = SR_WG * SR_BINS;         // 1024 bins per round
+constexpr int SR_CHUNK = 1024;                    // atoms staged in LDS per pass
+constexpr int SR_PASS_LANES = 32;                 // lanes per output pass
+constexpr int SR_PASS_BINS = SR_PASS_LANES * SR_BINS;  // 128 bins per wave per pass
 
-__device__ __forceinline__ df two_sum(float a, float b) {
-    float s = a + b;
-    float bb = s - a;
-    float e = (a - (s - bb)) + (b - bb);
-    return {s, e};
-}
-
-__device__ __forceinline__ df fast_two_sum(float a, float b) {
-    float s = a + b;
-    float e = b - (s - a);
-    return {s, e};
-}
-
-__device__ __forceinline__ df df_add(df a, df b) {
-    df s = two_sum(a.hi, b.hi);
-    df t = two_sum(a.lo, b.lo);
-    s.lo += t.hi;
-    s = fast_two_sum(s.hi, s.lo);
-    s.lo += t.lo;
-    return fast_two_sum(s.hi, s.lo);
-}
-
-// exact value of a (hi, lo) pair: 2 x 24 bits fit a double
-__device__ __forceinline__ double df_value(df a) { return (double)a.hi + (double)a.lo; }
-
-// ----------------------------------------------------------------------------
-// stage: binned merge + compensated prefix (rect) / FP64 atom table (exp)
-// ----------------------------------------------------------------------------
-
-constexpr int STAGE_WG = 1024;   // 16 wave64
-constexpr int STAGE_BINS = 8;    // consecutive bins per thread per round
-constexpr int STAGE_ROUND = STAGE_WG * STAGE_BINS;
-
-__device__ __forceinline__ df shfl_up_df(df v, int d) {
-    return {__shfl_up(v.hi, d), __shfl_up(v.lo, d)};
-}
-
-__device__ __forceinline__ df df_from(float x) { return {x, 0.0f}; }
-
-// One workgroup per template, rounds of 8192 bins, 8 consecutive bins per
-// thread.  A bin's merged value is the float32 sum over detectors X = 0,1,...
-// (in that order) of the detector's atoms in the bin (in atom order), starting
-// from 0 -- the same additions, in the same order, as lalpulsar's binned
-// merge, so the merged atoms are bit-identical.  Rect templates then get the
-// exclusive prefix P[b] = sum of bins < b per column as a double-float (FP32
-// two-sum) scan: running sums over the thread's 8 bins, a Kogge-Stone scan of
-// the thread totals within the wave, one thread per column scanning the 16
-// wave totals (plus the carry of earlier rounds) in LDS.  Each P[b] is stored
-// as the exact FP64 value hi + lo, [N+1][7].  Exp templates get the merged
-// atoms as an FP64 table [N][8].
-__global__ __launch_bounds__(STAGE_WG) void stage_kernel(
+// Rectangular-window templates: the exclusive prefix P[b] = sum of merged
+// bins < b per column, [N+1][7] doubles.  One workgroup per template, rounds
+// of 1024 bins, 4 consecutive bins per thread.
+//  - merge: per detector, the workgroup copies the round's atoms into LDS in
+//    chunks of 1024 with coalesced loads (a cursor per detector carries the
+//    first atom of the next round); each thread finds its first bin's atoms
+//    by binary search in LDS and sums its bins.  A bin's merged value is the
+//    float32 sum over detectors X = 0,1,... (in that order) of the detector's
+//    atoms in the bin (in atom order), from 0 -- the additions of
+//    lalpulsar's binned merge, in its order, so merged atoms are
+//    bit-identical;
+//  - scan (FP64): thread totals, a Kogge-Stone scan within the wave, the 4
+//    wave totals and the carry of earlier rounds in LDS.  A prefix value
+//    carries ~20 roundings of 2^-53 relative to the largest partial sum;
+//  - output: the LDS buffer is reused to transpose the prefix values, 32 lanes
+//    of each wave per pass, so every global store is a contiguous run of the
+//    [N+1][7] table (a thread's own 4 bins span 224 bytes).
+__global__ __launch_bounds__(SR_WG) void stage_kernel(
     const TmplDev *__restrict__ tds, const SegDev *__restrict__ segs,
     const uint32_t *__restrict__ ts, const float *__restrict__ cols, int64_t cstride,
-    double *__restrict__ prefix, double *__restrict__ merged) {
+    double *__restrict__ prefix) {
     const TmplDev td = tds[blockIdx.x];
-    const bool want_prefix = td.mode == MODE_RECT_FAST || td.mode == MODE_RECT_GENERIC;
-    if (!want_prefix) return;  // exp templates: stage_exp_kernel (block-uniform)
-    const uint32_t N = td.N, TA = td.TAtom;
+    if (td.mode != MODE_RECT_FAST && td.mode != MODE_RECT_GENERIC) return;  // block-uniform
+    // staging: ts [SR_CHUNK] uint32, then NC float planes [SR_CHUNK];
+    // output: per wave SR_PASS_BINS x NC doubles
+    __shared__ double s_buf[SR_CHUNK * (NC + 1) / 2];
+    static_assert(4 * SR_PASS_BINS * NC <= SR_CHUNK * (NC + 1) / 2, "output buffer");
+    uint32_t *s_ts = reinterpret_cast<uint32_t *>(s_buf);
+    float *s_x = reinterpret_cast<float *>(s_buf) + SR_CHUNK;
+    __shared__ double s_off[NC][SR_WG / 64];  // wave totals, then wave offsets
+    __shared__ double s_carry[NC];            // sum of all earlier rounds
+    __shared__ int64_t s_ka;
+
+    const uint32_t N = td.N;
+    const uint64_t TA = td.TAtom;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int nseg = (int)(td.seg_end - td.seg_begin);
-    __shared__ float sh_hi[NC][STAGE_WG / 64 + 1];  // wave totals, then offsets
-    __shared__ float sh_lo[NC][STAGE_WG / 64 + 1];  // [.][16] = running carry
-    if (tid < NC) {
-        sh_hi[tid][STAGE_WG / 64] = 0.0f;
-        sh_lo[tid][STAGE_WG / 64] = 0.0f;
-    }
-    double *P = prefix + td.prefix_off;
-    double *Xt = merged + td.merged_off;
+    double *__restrict__ P = prefix + td.prefix_off;
+    // thread X < nseg keeps detector X's cursor: its first atom not yet merged
+    int64_t cur = tid < nseg ? segs[td.seg_begin + tid].begin : 0;
+    if (tid < NC) s_carry[tid] = 0.0;
 
-    for (uint32_t base = 0; base < N; base += STAGE_ROUND) {
-        const uint32_t b0 = base + (uint32_t)tid * STAGE_BINS;
-        float v[STAGE_BINS][NC];
+    for (uint32_t base = 0; base < N; base += SR_ROUND) {
+        const uint32_t bend = min(N, base + (uint32_t)SR_ROUND);
+        const uint32_t b0 = base + (uint32_t)tid * SR_BINS;
+        const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b0 * TA;
+        const uint64_t tb_hi = (uint64_t)td.t_min + (uint64_t)bend * TA;  // later rounds' atoms
+        float v[SR_BINS][NC];
 #pragma unroll
-        for (int j = 0; j < STAGE_BINS; j++)
+        for (int j = 0; j < SR_BINS; j++)
 #pragma unroll
             for (int c = 0; c < NC; c++) v[j][c] = 0.0f;
-        if (b0 < N) {
-            const uint64_t t_lo = (uint64_t)td.t_min + (uint64_t)b0 * TA;
-            for (int X = 0; X < nseg; X++) {
-                const SegDev sg = segs[td.seg_begin + X];
-                int64_t lo = sg.begin, hi = sg.end;  // first atom with ts >= t_lo
+        for (int X = 0; X < nseg; X++) {
+            const int64_t seg_end = segs[td.seg_begin + X].end;
+            if (tid == X) s_ka = cur;
+            __syncthreads();
+            int64_t c0 = s_ka;
+            __syncthreads();  // s_ka is rewritten for the next detector
+            int64_t next = seg_end;
+            while (c0 < seg_end) {  // block-uniform
+                const int na = (int)min((int64_t)SR_CHUNK, seg_end - c0);
+                for (int i = tid; i < na; i += SR_WG) {
+                    s_ts[i] = ts[c0 + i];
+#pragma unroll
+                    for (int c = 0; c < NC; c++) s_x[c * SR_CHUNK + i] = cols[c * cstride + c0 + i];
+                }
+                __syncthreads();
+                if (b0 < bend) {
+                    int lo = 0, hi = na;  // first staged atom with ts >= t_lo
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if ((uint64_t)s_ts[mid] < t_lo)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    int k = lo;
+#pragma unroll
+                    for (int j = 0; j < SR_BINS; j++) {
+                        const uint64_t t_hi = t_lo + (uint64_t)(j + 1) * TA;
+                        for (; k < na && (uint64_t)s_ts[k] < t_hi; k++) {
+#pragma unroll
+                            for (int c = 0; c < NC; c++) v[j][c] += s_x[c * SR_CHUNK + k];
+                        }
+                    }
+                }
+                int lo = 0, hi = na;  // first staged atom of a later round (same on all threads)
                 while (lo < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if ((uint64_t)ts[mid] < t_lo)
+                    const int mid = (lo + hi) >> 1;
+                    if ((uint64_t)s_ts[mid] < tb_hi)
                         lo = mid + 1;
                     else
                         hi = mid;
                 }
-                int64_t k = lo;
-#pragma unroll
-                for (int j = 0; j < STAGE_BINS; j++) {
-                    const uint64_t t_hi = t_lo + (uint64_t)(j + 1) * TA;
-                    for (; k < sg.end && (uint64_t)ts[k] < t_hi; k++) {
-#pragma unroll
-                        for (int c = 0; c < NC; c++) v[j][c] += cols[c * cstride + k];
-                    }
+                __syncthreads();  // staged atoms consumed before the next chunk
+                if (lo < na) {
+                    next = c0 + lo;
+                    break;
                 }
+                c0 += na;
             }
+            if (tid == X) cur = next;
         }
-        if (!want_prefix) {
-#pragma unroll
-            for (int j = 0; j < STAGE_BINS; j++) {
-                const uint32_t b = b0 + j;
-                if (b < N) {
-#pragma unroll
-                    for (int c = 0; c < NC; c++) Xt[(int64_t)b * MERGED_STRIDE + c] = (double)v[j][c];
-                    Xt[(int64_t)b * MERGED_STRIDE + NC] = 0.0;
-                }
-            }
-            continue;
-        }
-        // thread total (bins beyond N hold zeros)
-        df x[NC];
+        // inclusive scan of the thread totals within the wave (bins >= N hold 0)
+        double x[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            x[c] = df_from(v[0][c]);
+            double t = 0.0;
 #pragma unroll
-            for (int j = 1; j < STAGE_BINS; j++) x[c] = df_add(x[c], df_from(v[j][c]));
+            for (int j = 0; j < SR_BINS; j++) t += (double)v[j][c];
+            x[c] = t;
         }
-        // inclusive scan of thread totals within the wave
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const df y = shfl_up_df(x[c], d);
-                if (lane >= d) x[c] = df_add(y, x[c]);
+                const double y = __shfl_up(x[c], d);
+                if (lane >= d) x[c] += y;
             }
         }
         if (lane == 63) {
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
-                sh_hi[c][wave] = x[c].hi;
-                sh_lo[c][wave] = x[c].lo;
-            }
+            for (int c = 0; c < NC; c++) s_off[c][wave] = x[c];
         }
         __syncthreads();
         if (tid < NC) {  // exclusive scan of the wave totals, seeded by the carry
-            df run = {sh_hi[tid][STAGE_WG / 64], sh_lo[tid][STAGE_WG / 64]};
-            for (int w = 0; w < STAGE_WG / 64; w++) {
-                const df t = {sh_hi[tid][w], sh_lo[tid][w]};
-                sh_hi[tid][w] = run.hi;
-                sh_lo[tid][w] = run.lo;
-                run = df_add(run, t);
+            double run = s_carry[tid];
+            for (int w = 0; w < SR_WG / 64; w++) {
+                const double t = s_off[tid][w];
+                s_off[tid][w] = run;
+                run += t;
             }
-            sh_hi[tid][STAGE_WG / 64] = run.hi;
-            sh_lo[tid][STAGE_WG / 64] = run.lo;
+            s_carry[tid] = run;
         }
         __syncthreads();
+        double ex[NC];  // exclusive prefix at the thread's first bin
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            df ex = shfl_up_df(x[c], 1);  // exclusive within the wave
-            if (lane == 0) ex = {0.0f, 0.0f};
-            df run = df_add(df{sh_hi[c][wave], sh_lo[c][wave]}, ex);
-#pragma unroll
-            for (int j = 0; j < STAGE_BINS; j++) {
-                const uint32_t b = b0 + j;
-                if (b < N) P[(int64_t)b * NC + c] = df_value(run);
-                run = df_add(run, df_from(v[j][c]));
-            }
+            double e = __shfl_up(x[c], 1);
+            if (lane == 0) e = 0.0;
+            ex[c] = s_off[c][wave] + e;
         }
-        __syncthreads();  // offsets consumed before the next round rewrites them
+        double *out = s_buf + wave * (SR_PASS_BINS * NC);
+#pragma unroll 1
+        for (int pass = 0; pass < 64 / SR_PASS_LANES; pass++) {
+            if (lane / SR_PASS_LANES == pass && b0 < bend) {
+                const int q = lane % SR_PASS_LANES;
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    double r = ex[c];
+#pragma unroll
+                    for (int j = 0; j < SR_BINS; j++) {
+                        out[(q * SR_BINS + j) * NC + c] = r;
+                        r += (double)v[j][c];
+                    }
+                }
+            }
+            __syncthreads();
+            const int64_t wb = (int64_t)base + wave * (64 * SR_BINS) + pass * SR_PASS_BINS;
+            const int cnt = (int)max((int64_t)0, min((int64_t)SR_PASS_BINS, (int64_t)bend - wb));
+            for (int i = lane; i < cnt * NC; i += 64) P[wb * NC + i] = out[i];
+            __syncthreads();  // copied out before the next pass (or round) rewrites
+        }
     }
-    if (want_prefix && tid < NC)
-        P[(int64_t)N * NC + tid] = df_value(df{sh_hi[tid][STAGE_WG / 64], sh_lo[tid][STAGE_WG / 64]});
+    if (tid < NC) P[(int64_t)N * NC + tid] = s_carry[tid];
 }
 
 // Exponential-window templates: the merged atoms as the FP64 table [N][8]
@@ -411,8 +415,10 @@ __device__ __forceinline__ uint32_t idx_end_q(uint32_t q, uint32_t N) {
 
 constexpr float LOG2E = 1.4426950408889634f;
 
-__device__ __forceinline__ float exp_rel(float x, float ref) {
-    return __builtin_amdgcn_exp2f((x - ref) * LOG2E);  // exp(x - ref); exp(-inf) = 0
+// 2^(aL - bL) = exp(a - b) for references kept in log2 units (aL = a*LOG2E);
+// exp2(-inf) = 0
+__device__ __forceinline__ float exp2_diff(float aL, float bL) {
+    return __builtin_amdgcn_exp2f(aL - bL);
 }
 
 
@@ -520,14 +526,17 @@ __device__ __forceinline__ float rows_sum<4>(const float (&v)[4], int lane, int 
 // R wave-uniform rows x those 4 columns, with invalid cells set to -inf.
 //  - argmax: lane maximum of the sub-tile; only when it beats the lane's best
 //    is the first (row-major) position searched;
-//  - BtSG: sum of exp(F - Mw), Mw the wave's running maximum (accumulators
-//    rescaled when it grows); column sums stay in registers across sub-tiles,
-//    row sums are wave-reduced and parked in LDS with their Mw.
+//  - BtSG: sum of 2^(F*LOG2E - ML), ML = Mw*LOG2E rounded to float, Mw the
+//    wave's running maximum (accumulators rescaled when it grows) -- one FMA
+//    and one exp2 per cell; each sum is relative to its reference in log2
+//    units (ML), i.e. to exp(F - ML/LOG2E).  Column sums stay in registers
+//    across sub-tiles, row sums are wave-reduced and parked in LDS with ML.
 template <bool BTSG, int NCOLS = 4>
 struct Reducer {
     float best;
     int bidx;
     float Mw;
+    float ML;
     float Stot;
     float Scol[NCOLS];
 
@@ -535,6 +544,7 @@ struct Reducer {
         best = -INFINITY;
         bidx = 0x7fffffff;
         Mw = -INFINITY;
+        ML = -INFINITY;
         Stot = 0.0f;
 #pragma unroll
         for (int c = 0; c < NCOLS; c++) Scol[c] = 0.0f;
@@ -574,11 +584,13 @@ struct Reducer {
         // sees a value more than REF_SLACK above it
         if (__any(lm > Mw + REF_SLACK)) {
             const float wm = wave_max(lm);
-            const float sc = exp_rel(Mw, wm);
+            const float wmL = wm * LOG2E;
+            const float sc = exp2_diff(ML, wmL);
             Stot *= sc;
 #pragma unroll
             for (int c = 0; c < NCOLS; c++) Scol[c] *= sc;
             Mw = wm;
+            ML = wmL;
         }
         if (Mw == -INFINITY) return;  // wave-uniform: no valid cell yet
         float sr[R];
@@ -587,7 +599,7 @@ struct Reducer {
             sr[r] = 0.0f;
 #pragma unroll
             for (int c = 0; c < NCOLS; c++) {
-                const float e = exp_rel(F[r][c], Mw);
+                const float e = __builtin_amdgcn_exp2f(fmaf(F[r][c], LOG2E, -ML));
                 sr[r] += e;
                 Scol[c] += e;
             }
@@ -596,7 +608,7 @@ struct Reducer {
         int row;
         const float srw = rows_sum<R>(sr, lane, row);
         if (row >= 0) {
-            sh_rowM[rl0 + row] = Mw;
+            sh_rowM[rl0 + row] = ML;
             sh_rowS[rl0 + row] = srw;
         }
     }
@@ -604,7 +616,8 @@ struct Reducer {
 
 // End of a work item: combine the 4 waves, then write the item partial
 // {max, argmax, M, S} and, for BtSG, the row and column partial sums rescaled
-// to the item maximum M (so each is a single float).
+// to the item reference M (so each is a single float).  M is in log2 units:
+// the partial sums are relative to exp(M / LOG2E).
 template <bool BTSG, int NCOLS, typename ColMap>
 __device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const TmplDev &td,
                             const ItemDev &it, int64_t item_id, int nrows_item,
@@ -622,7 +635,7 @@ __device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const
     if (lane == 0) {
         sh_best[wave] = red.best;
         sh_bidx[wave] = red.bidx;
-        sh_M[wave] = red.Mw;
+        sh_M[wave] = red.ML;
         sh_S[wave] = stot;
     }
     if (BTSG) {
@@ -638,13 +651,13 @@ __device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const
             float S = 0.0f;
 #pragma unroll
             for (int w = 0; w < 4; w++)
-                if (sh_M[w] != -INFINITY) S += sh_col[w][tid] * exp_rel(sh_M[w], M);
+                if (sh_M[w] != -INFINITY) S += sh_col[w][tid] * exp2_diff(sh_M[w], M);
             colpart[td.colpart_off + (int64_t)it.rb * td.N_tau + n] = S;
         }
         for (int rl = tid; rl < nrows_item; rl += WG) {
             const int m = it.rb * td.rb_rows + rl;
             rowpart[td.rowpart_off + (int64_t)it.cb * td.N_t0 + m] =
-                sh_rowS[rl] * exp_rel(sh_rowM[rl], M);
+                sh_rowS[rl] * exp2_diff(sh_rowM[rl], M);
         }
     }
     if (tid == 0) {
@@ -656,7 +669,7 @@ __device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const
         if (BTSG) {
 #pragma unroll
             for (int w = 0; w < 4; w++)
-                if (sh_M[w] != -INFINITY) S += sh_S[w] * exp_rel(sh_M[w], M);
+                if (sh_M[w] != -INFINITY) S += sh_S[w] * exp2_diff(sh_M[w], M);
         }
         ipart[item_id] = ItemPart{bv, bi, M, S};
     }
@@ -1358,7 +1371,7 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     if (lane == 0) {
         sh_best[wave] = red.best;
         sh_bidx[wave] = red.bidx;
-        sh_M[wave] = red.Mw;
+        sh_M[wave] = red.ML;
         sh_S[wave] = stot;
     }
     __syncthreads();
@@ -1366,13 +1379,13 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     if (BTSG) {
         if (valid)
             colpart[td.colpart_off + (int64_t)it.rb * td.N_tau + n] =
-                red.Mw == -INFINITY ? 0.0f : red.Scol[0] * exp_rel(red.Mw, M);
+                red.ML == -INFINITY ? 0.0f : red.Scol[0] * exp2_diff(red.ML, M);
         const int nrows = m_hi - row0 + 1;
         for (int rl = threadIdx.x; rl < nrows; rl += WG) {
             float S = 0.0f;
 #pragma unroll
             for (int w = 0; w < 4; w++)
-                if (sh_M[w] != -INFINITY) S += sh_rowS[w][rl] * exp_rel(sh_rowM[w][rl], M);
+                if (sh_M[w] != -INFINITY) S += sh_rowS[w][rl] * exp2_diff(sh_rowM[w][rl], M);
             rowpart[td.rowpart_off + (int64_t)it.cb * td.N_t0 + row0 + rl] = S;
         }
     }
@@ -1385,7 +1398,7 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
         if (BTSG) {
 #pragma unroll
             for (int w = 0; w < 4; w++)
-                if (sh_M[w] != -INFINITY) S += sh_S[w] * exp_rel(sh_M[w], M);
+                if (sh_M[w] != -INFINITY) S += sh_S[w] * exp2_diff(sh_M[w], M);
         }
         ipart[item_id] = ItemPart{bv, bi, M, S};
     }
@@ -1482,17 +1495,23 @@ __global__ __launch_bounds__(NT) void finalize_kernel(
     double lnB = NAN, t0_MP = NAN, tau_MP = NAN;
     int64_t m_MP = -1, n_MP = -1;
     if (BTSG) {
-        // item k's partials are relative to its maximum M_k <= maxF
+        // item k's partials are relative to exp(M_k / LOG2E), M_k in log2
+        // units and M_k / LOG2E <= maxF up to rounding
+        constexpr double INV_LOG2E = 1.0 / (double)LOG2E;
+        // M_k / LOG2E - maxF (FP64; small for the items that matter)
+        auto item_arg = [&](int64_t k) -> double {
+            return fma((double)ip[k].M, INV_LOG2E, -(double)maxF);
+        };
         double s = 0.0;
         for (int64_t k = threadIdx.x; k < nitems; k += NT) {
-            const float sc = exp_rel(ip[k].M, maxF);
-            if (k < FIN_SCALE_CAP) sh_scale[k] = sc;
-            s += (double)ip[k].S * exp((double)ip[k].M - (double)maxF);
+            const double arg = item_arg(k);
+            if (k < FIN_SCALE_CAP) sh_scale[k] = __expf((float)arg);
+            s += (double)ip[k].S * exp(arg);
         }
         s = block_sum_d<NT>(s);  // contains a barrier: sh_scale is complete
         lnB = log(70.0 / ((double)td.N_t0 * (double)td.N_tau)) + (double)maxF + log(s);
         auto scale = [&](int64_t k) -> float {
-            return k < FIN_SCALE_CAP ? sh_scale[k] : exp_rel(ip[k].M, maxF);
+            return k < FIN_SCALE_CAP ? sh_scale[k] : __expf((float)item_arg(k));
         };
         double rv = -1.0;
         int ri = 0x7fffffff;
@@ -1755,6 +1774,11 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
     int64_t N_t0, N_tau;
     int rc = effective_window(N, TAtom, t_min, win_in, w, N_t0, N_tau);
     if (rc) return rc;
+    if (seg_end - seg_begin > SR_WG)  // the merge keeps one cursor per detector and thread
+        return tcwc::fail(TCW_ERR_INVALID, "template " + std::to_string(t) + " has " +
+                                           std::to_string(seg_end - seg_begin) +
+                                           " detector segments; at most " +
+                                           std::to_string(SR_WG) + " are supported.");
     TmplDev td;
     std::memset(&td, 0, sizeof(td));
     td.seg_begin = seg_begin;
@@ -1991,9 +2015,9 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)((p.T + 255) / 256)), dim3(256), 0, st,
                        flags, p.T);
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[0], st));
-    hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(STAGE_WG), 0, st,
+    hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(SR_WG), 0, st,
                        (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
-                       cstride, (double *)p.d_prefix.p, (double *)p.d_merged.p);
+                       cstride, (double *)p.d_prefix.p);
     if (!p.exp_tmpls.empty())
         hipLaunchKernelGGL(stage_exp_kernel,
                            dim3((unsigned)((p.exp_max_N + SX_BINS_BLOCK - 1) / SX_BINS_BLOCK),
