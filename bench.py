@@ -561,7 +561,7 @@ def main():
             "BtSG": BtSG,
             "hipgraph": use_graph,
             "rows_per_item_by_mode": tuned,
-            "arithmetic": "fp32 two-sum prefix scan, fp64 window differences/determinant/"
+            "arithmetic": "fp64 prefix scan of float32 merged bins, fp64 window differences/determinant/"
                           "normalisation, fp32 map",
         },
         "roofline": {
