@@ -65,6 +65,15 @@ PROFILED_TRAFFIC = {
     "exp_shared": (0.101e9 + 0.543e9, "profiles/exp_fast_kernel.exp_shared.md"),
 }
 FP64_VALU_PEAK_TFLOPS = 78.6
+# What bounds the map kernel instead of HBM (rocprofv3 counters of the same
+# kernel, summarised in the profile files named here).
+LIMITER = {
+    "rect_batched": "FP64 VALU issue: ~45 VALU instructions per cell, SQ_ACTIVE_INST_VALU ~95% of "
+                    "SIMD cycles at the engine clock the card holds under this load (GRBM_GUI_ACTIVE: "
+                    "~1.93 GHz, not 2.4); profiles/rect_fast_kernel.rect_batched.md",
+    "exp_shared": "FP64 VALU issue (recurrence step + F per cell); "
+                  "profiles/exp_fast_kernel.exp_shared.md",
+}
 
 
 def make_workload(args):
@@ -578,6 +587,7 @@ def main():
             "stage_ms": stage_ms[0],
             "finalize_ms": stage_ms[2],
             "job_GBps": job_bytes / (ms_per_step / 1e3) / 1e9,
+            "limiter": LIMITER.get(args.workload),
         },
         "cpu_baseline": cpu,
         "plain": plain,

@@ -192,6 +192,56 @@ def test_merge_offgrid_and_duplicate_bins(hip_ready):
         _check_full(multi, win)
 
 
+def _long_multi(seed=31, n_bins=3000):
+    """3 detectors over ~n_bins TAtom slots, 10-30% gaps, a second atom in one
+    bin of seven, detector start offsets inside the first bin."""
+    from pyfstat_amd.atoms import FstatAtomVector, MultiFstatAtomVector
+
+    rng = np.random.default_rng(seed)
+    multi = MultiFstatAtomVector(3)
+    for X in range(3):
+        slots = np.nonzero(synth.gap_mask(rng, n_bins, 0.1 + 0.1 * X))[0]
+        dup = rng.choice(slots, size=len(slots) // 7, replace=False)
+        t = np.sort(np.concatenate([slots * synth.TSFT, dup * synth.TSFT + 600]))
+        t = (t + synth.T_START + 13 * X).astype(np.int64)
+        cols = synth.detector_atoms(rng, t.astype(np.float64), 0.3 * X, 1.1 + X,
+                                    np.zeros(t.size, np.complex128))
+        multi.data[X] = FstatAtomVector.from_arrays(synth.TSFT, t, cols[0], cols[1], cols[2],
+                                                    cols[3] + 1j * cols[4],
+                                                    cols[5] + 1j * cols[6])
+    return multi
+
+
+@pytest.mark.parametrize("force_generic", [False, True])
+def test_rect_long_multi_detector_merge(hip_ready, force_generic):
+    """~3000 bins and ~2600-3000 atoms per detector: the merge and prefix scan
+    run over several rounds of bins and several LDS chunks of atoms per
+    detector; window starts and ends cover the whole span (dt0 = dtau = 37
+    TAtom, so the regular-grid kernel takes it unless forced generic)."""
+    multi = _long_multi()
+    win = synth.grid_window(TRANSIENT_RECTANGULAR, synth.T_START, 75, 70, dt0=37 * synth.TSFT,
+                            dtau=37 * synth.TSFT)
+    _check_full(multi, win, force_generic=force_generic)
+
+
+def test_batched_rect_long_templates(hip_ready):
+    """Ragged batch of templates with 1500-2600 SFTs (up to three rounds of
+    the prefix scan each) against the oracle."""
+    from pyfstat_amd.batched import BatchedTransientMaps
+
+    batch = synth.workload_rect_batched(T=3, n_min=1500, n_max=2600, seed_index=13)
+    b = BatchedTransientMaps.from_synthetic(batch, BtSG=True)
+    b.run()
+    maps = b.maps()
+    for t in range(batch.T):
+        print(f"long batched template {t}", flush=True)
+        om = oracle_for(batch.template(t), batch.windows[t])
+        compare_maps(maps[t].F_mn, om)
+        assert abs(maps[t].lnBtSG - om.lnBtSG) <= lnb_tol(om.maxF)
+        _posterior_ok(om, batch.windows[t], maps[t].t0_MP, maps[t].tau_MP)
+    b.close()
+
+
 def test_lal_strict_single_atom(hip_ready):
     """lal-strict mode raises where lalpulsar raises XLAL_EDOM; default mode
     returns the F = 2 fallback there."""
