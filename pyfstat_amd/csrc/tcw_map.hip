@@ -1148,6 +1148,30 @@ __device__ __forceinline__ void expf_step1(const ExpWalk &wk, const ExpCol &ec, 
     }
 }
 
+// The same step with the entering atom kh = k - 1 (the same on every lane)
+// read from the FP64 table through scalar loads: an SGPR operand of the FMA,
+// no LDS read or conversion.  Used by the walk that evaluates no F
+// (exp_seg_kernel); in the map kernel the scalar loads would share the LGKM
+// wait counter with its LDS reads and serialise them (measured 11% slower).
+__device__ __forceinline__ void expf_step1_sh(const ExpWalk &wk, const ExpCol &ec, int64_t kh,
+                                              int jt, double (&E)[NC]) {
+    double xh[NC];
+    if (kh < wk.H) {  // wave-uniform; atoms at or past H count as zero
+        const double *__restrict__ ph = wk.X + kh * MERGED_STRIDE;
+#pragma unroll
+        for (int c = 0; c < NC; c++) xh[c] = ph[c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; c++) xh[c] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int g = c < 3 ? 0 : 1;
+        const double xt = (double)wk.lds[c * EXPF_LDS_CAP + jt];
+        E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh[c]));
+    }
+}
+
 // Etab layout per template: [segment][component][tau column]
 __device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, int s, int c) {
     return etab + td.etab_off + ((int64_t)s * NC + c) * td.N_tau;
@@ -1177,12 +1201,12 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
     for (int c = 0; c < NC; c++) E[c] = 0.0;
     int k = td.k00 + td.a * r_top;
     if (wk.in_lds && td.a == 1) {  // block-uniform
-        int jh = (int)(k - 1 - wk.head_lo);
+        int64_t kh = (int64_t)k - 1;
         int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
 #pragma unroll 2
         for (int m = r_top; m > r_bot; m--) {
-            expf_step1(wk, ec, jh, jt, E);
-            jh--;
+            expf_step1_sh(wk, ec, kh, jt, E);
+            kh--;
             jt--;
         }
     } else {
@@ -1202,12 +1226,16 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
 // the walks), split over INIT_PARTS blocks per (template, column block): part
 // p sums its share of l into slot n_seg + p; the scan adds the parts.  One
 // lane per tau column.  The l range is block-uniform (from the block's longest
-// window), so the atom loads are scalar; a lane's weight drops to zero past its
-// own window length.
+// window): the block copies its atoms into LDS in chunks with coalesced loads
+// and the lanes read the same row (a broadcast), each up to its own window
+// length.
+constexpr int INIT_CHUNK = 256;  // atoms per LDS chunk: 256 x 64 B = 16 KB
+
 __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict__ tds,
                                                       const ItemDev *__restrict__ items,
                                                       const double *__restrict__ merged,
                                                       double *__restrict__ etab) {
+    __shared__ double s_x[INIT_CHUNK * MERGED_STRIDE];
     const ItemDev it = items[blockIdx.x / INIT_PARTS];
     const int part = (int)(blockIdx.x % INIT_PARTS);
     const TmplDev td = tds[it.t];
@@ -1226,19 +1254,30 @@ __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict_
     double E[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) E[c] = 0.0;
-    if (l0 < l1) {
+    if (l0 < l1) {  // block-uniform
         const double tau = (double)td.tau + (double)(valid ? n : col_last) * (double)td.dtau;
         const double w1 = exp(-(double)l0 * (double)td.TAtom / tau);  // rho^l0
         double w[2] = {w1 * w1, w1};
+        // rows k0 + l, l < l1 <= room, lie below H <= N
         const double *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
+        for (int64_t c0 = l0; c0 < l1; c0 += INIT_CHUNK) {
+            const int na = (int)min((int64_t)INIT_CHUNK, l1 - c0);
+            __syncthreads();  // the previous chunk is consumed
+            for (int i = threadIdx.x; i < na * MERGED_STRIDE; i += WG)
+                s_x[i] = X[c0 * MERGED_STRIDE + i];
+            __syncthreads();
+            // this lane's window ends at lr: a wave whose lanes are all past
+            // their windows leaves the loop (window lengths grow with n, so
+            // the early waves of a block finish first)
+            const int jn = (int)max((int64_t)0, min((int64_t)na, lr - c0));
 #pragma unroll 4
-        for (int64_t l = l0; l < l1; l++) {
-            if (l == lr) w[0] = w[1] = 0.0;  // past this lane's window
+            for (int j = 0; j < jn; j++) {
 #pragma unroll
-            for (int c = 0; c < NC; c++)
-                E[c] = fma(w[c < 3 ? 0 : 1], X[l * MERGED_STRIDE + c], E[c]);
-            w[0] *= ec.rho[0];
-            w[1] *= ec.rho[1];
+                for (int c = 0; c < NC; c++)
+                    E[c] = fma(w[c < 3 ? 0 : 1], s_x[j * MERGED_STRIDE + c], E[c]);
+                w[0] *= ec.rho[0];
+                w[1] *= ec.rho[1];
+            }
         }
     }
     if (valid) {
