@@ -136,7 +136,8 @@ def semicoherent_twoF(multi_list, window, singleFstats=False, BSGLSetup=None,
 
 def transient_detstats(multi_list, windows, BtSG=False, singleFstats=False, BSGLSetup=None,
                        use_graph=False):
-    """maxTwoF (NaN -> 0), lnBtSG, t0_ML/tau_ML for every template and, with
+    """maxTwoF (NaN -> 0), lnBtSG, t0_ML/tau_ML (and the argmax indices m_ML,
+    n_ML) for every template and, with
     singleFstats, twoFXatMaxTwoF [T, X]: each detector's 2F at the cell that
     maximises the multi-detector F (core.py:1447-1471, 1503-1545), all maps in
     one device pass."""
@@ -157,6 +158,8 @@ def transient_detstats(multi_list, windows, BtSG=False, singleFstats=False, BSGL
             "maxTwoF": maxTwoF,
             "t0_ML": st["t0_ML"][:T].copy(),
             "tau_ML": st["tau_ML"][:T].copy(),
+            "m_ML": st["m_ML"][:T].copy(),
+            "n_ML": st["n_ML"][:T].copy(),
         }
         if BtSG:
             out["lnBtSG"] = st["lnBtSG"][:T].copy()
