@@ -1056,46 +1056,62 @@ constexpr int MAX_RB_EXPF = 128;
 // above the map instead of summing the top chunk directly
 constexpr int64_t DIRECT_INIT_MIN_BLOCKS = 256;
 constexpr int INIT_PARTS = 8;  // blocks sharing one column block's direct sum
-// atoms staged in LDS as float32 (the merged atoms are float32 sums, so the
-// copy is exact), 7 planes of EXPF_LDS_CAP: 1088 * 28 B = 30.5 KB, so several
-// workgroups share a CU
-constexpr int EXPF_LDS_CAP = 1088;
+// Atoms a walk stages in LDS.  Entering (head) atoms are the same for every
+// lane of a row step: FP64 planes of EXPF_HEAD_CAP, read as broadcasts with
+// no conversion.  Leaving (tail) atoms differ per lane (L grows with the
+// column): float32 planes of EXPF_TAIL_CAP (the merged atoms are float32
+// sums, so the copy is exact), half the bytes.  7 x (960 x 4 + 128 x 8) B =
+// 33.3 KB, so four workgroups share a CU.  Plane strides are multiples of 64
+// elements, so two planes are one ds_read2st64.
+constexpr int EXPF_TAIL_CAP = 960;
+constexpr int EXPF_HEAD_CAP = MAX_RB_EXPF;
+constexpr int EXPF_LDS_BYTES = NC * (EXPF_TAIL_CAP * (int)sizeof(float) +
+                                     EXPF_HEAD_CAP * (int)sizeof(double));
 
-// Atoms a walk from row r_top down to row r_bot touches: entering atoms
-// [k(r_bot), k(r_top)) and leaving atoms shifted by L(n), n in [col0, col_last].
-__host__ __device__ __forceinline__ int expf_stage_atoms(int a, int rows, int Ls, int cols) {
+// Atoms a walk over `rows` rows touches: entering atoms [k(r_bot), k(r_top))
+// (nhead = a (rows - 1)) and leaving atoms shifted by L(n), n in [col0,
+// col_last] (ntail = nhead + Ls (cols - 1)).
+__host__ __device__ __forceinline__ bool expf_fits_lds(int a, int rows, int Ls, int cols) {
     const int nhead = a * (rows - 1);
-    return nhead + nhead + Ls * (cols - 1);
+    return nhead <= EXPF_HEAD_CAP && nhead + Ls * (cols - 1) <= EXPF_TAIL_CAP;
 }
 
 struct ExpWalk {
     const double *X;
-    float *lds;  // atom planes [c][EXPF_LDS_CAP]; merged atoms are float32 sums, so exact
+    float *lds;     // tail planes [c][EXPF_TAIL_CAP]
+    double *ldsh;   // head planes [c][EXPF_HEAD_CAP]
     int64_t head_lo, tail_lo;
-    int nhead, stride;  // LDS rows: [c][stride], head atoms then tail atoms
+    int nhead, ntail;
     int H;
     bool in_lds;
 };
 
+// lds_ok: the launch carries the EXPF_LDS_BYTES of dynamic LDS
 __device__ __forceinline__ ExpWalk expf_stage(const TmplDev &td, const double *X, float *lds,
-                                              int lds_atoms, int r_top, int r_bot, int col0,
+                                              int lds_ok, int r_top, int r_bot, int col0,
                                               int col_last) {
     ExpWalk wk;
     wk.X = X;
     wk.lds = lds;
+    wk.ldsh = reinterpret_cast<double *>(lds + NC * EXPF_TAIL_CAP);
     wk.H = td.H;
     wk.head_lo = (int64_t)td.k00 + (int64_t)td.a * r_bot;
     wk.nhead = td.a * (r_top - r_bot);
     wk.tail_lo = wk.head_lo + td.L0 + (int64_t)td.Ls * col0;
-    const int ntail = wk.nhead + td.Ls * (col_last - col0);
-    wk.stride = wk.nhead + ntail;  // staged atoms; LDS planes are EXPF_LDS_CAP apart
-    wk.in_lds = wk.stride <= lds_atoms;  // block-uniform
+    wk.ntail = wk.nhead + td.Ls * (col_last - col0);
+    wk.in_lds = lds_ok && wk.nhead <= EXPF_HEAD_CAP && wk.ntail <= EXPF_TAIL_CAP;  // uniform
     if (wk.in_lds) {
-        for (int idx = threadIdx.x; idx < wk.stride * NC; idx += WG) {
+        for (int idx = threadIdx.x; idx < wk.nhead * NC; idx += WG) {
             const int j = idx / NC;
             const int c = idx - j * NC;
-            const int64_t atom = j < wk.nhead ? wk.head_lo + j : wk.tail_lo + (j - wk.nhead);
-            lds[c * EXPF_LDS_CAP + j] = atom < wk.H ? (float)X[atom * MERGED_STRIDE + c] : 0.0f;
+            const int64_t atom = wk.head_lo + j;
+            wk.ldsh[c * EXPF_HEAD_CAP + j] = atom < wk.H ? X[atom * MERGED_STRIDE + c] : 0.0;
+        }
+        for (int idx = threadIdx.x; idx < wk.ntail * NC; idx += WG) {
+            const int j = idx / NC;
+            const int c = idx - j * NC;
+            const int64_t atom = wk.tail_lo + j;
+            lds[c * EXPF_TAIL_CAP + j] = atom < wk.H ? (float)X[atom * MERGED_STRIDE + c] : 0.0f;
         }
     }
     __syncthreads();
@@ -1111,11 +1127,11 @@ __device__ __forceinline__ void expf_step(const ExpWalk &wk, const ExpCol &ec, i
         double xh[NC], xt[NC];
         if (wk.in_lds) {
             const int jh = (int)(kk - wk.head_lo);
-            const int jt = wk.nhead + (int)(kt - wk.tail_lo);
+            const int jt = (int)(kt - wk.tail_lo);
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                xh[c] = (double)wk.lds[c * EXPF_LDS_CAP + jh];
-                xt[c] = (double)wk.lds[c * EXPF_LDS_CAP + jt];
+                xh[c] = wk.ldsh[c * EXPF_HEAD_CAP + jh];
+                xt[c] = (double)wk.lds[c * EXPF_TAIL_CAP + jt];
             }
         } else {
             const double *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
@@ -1135,15 +1151,16 @@ __device__ __forceinline__ void expf_step(const ExpWalk &wk, const ExpCol &ec, i
 }
 
 // The common case of one row step = one atom step (dt0 == TAtom) with the
-// atoms staged in LDS: jh / jt are the LDS slots of the entering atom
-// (k - 1) and of the leaving atom (k - 1 + L), both falling by one per row.
+// atoms staged in LDS: jh / jt are the head and tail LDS slots of the
+// entering atom (k - 1) and of the leaving atom (k - 1 + L), both falling by
+// one per row.
 __device__ __forceinline__ void expf_step1(const ExpWalk &wk, const ExpCol &ec, int jh, int jt,
                                            double (&E)[NC]) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int g = c < 3 ? 0 : 1;
-        const double xh = (double)wk.lds[c * EXPF_LDS_CAP + jh];
-        const double xt = (double)wk.lds[c * EXPF_LDS_CAP + jt];
+        const double xh = wk.ldsh[c * EXPF_HEAD_CAP + jh];
+        const double xt = (double)wk.lds[c * EXPF_TAIL_CAP + jt];
         E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh));
     }
 }
@@ -1167,7 +1184,7 @@ __device__ __forceinline__ void expf_step1_sh(const ExpWalk &wk, const ExpCol &e
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int g = c < 3 ? 0 : 1;
-        const double xt = (double)wk.lds[c * EXPF_LDS_CAP + jt];
+        const double xt = (double)wk.lds[c * EXPF_TAIL_CAP + jt];
         E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh[c]));
     }
 }
@@ -1181,7 +1198,7 @@ __device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, in
 // top row of chunk s-1) started from zero.  Work items (t, s, cb).
 __global__ __launch_bounds__(WG) void exp_seg_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items,
-    const double *__restrict__ merged, double *__restrict__ etab, int lds_atoms) {
+    const double *__restrict__ merged, double *__restrict__ etab, int lds_ok) {
     extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
     float *lds = reinterpret_cast<float *>(lds_raw);  // staged atoms, exact float32
     const ItemDev it = items[block_slot()];
@@ -1193,7 +1210,7 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
     const bool valid = n < N_tau;
     const int col_last = min(col0 + TILE_COLS, N_tau) - 1;
     const int r_top = (it.rb + 1) * RB - 1, r_bot = it.rb * RB - 1;
-    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_atoms, r_top, r_bot, col0,
+    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_ok, r_top, r_bot, col0,
                                   col_last);
     const ExpCol ec = exp_col(td, valid ? n : col0);
     double E[NC];
@@ -1202,7 +1219,7 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
     int k = td.k00 + td.a * r_top;
     if (wk.in_lds && td.a == 1) {  // block-uniform
         int64_t kh = (int64_t)k - 1;
-        int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
+        int jt = (int)(k - 1 + ec.L - wk.tail_lo);
 #pragma unroll 2
         for (int m = r_top; m > r_bot; m--) {
             expf_step1_sh(wk, ec, kh, jt, E);
@@ -1331,7 +1348,7 @@ __device__ __forceinline__ void expf_rows(const TmplDev &td, const ExpWalk &wk, 
     const int colidx[1] = {n};
     int k = td.k00 + td.a * r_top;
     int jh = (int)(k - 1 - wk.head_lo);
-    int jt = wk.nhead + (int)(k - 1 + ec.L - wk.tail_lo);
+    int jt = (int)(k - 1 + ec.L - wk.tail_lo);
     for (int g = r_top; g >= row0; g -= 4) {
         const bool last = g - 3 == row0;
         float F4[4][1];
@@ -1365,7 +1382,7 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
     const double *__restrict__ merged, const double *__restrict__ etab,
     float *__restrict__ Fout, ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
-    float *__restrict__ colpart, int *__restrict__ flags, int lds_atoms) {
+    float *__restrict__ colpart, int *__restrict__ flags, int lds_ok) {
     extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
     float *lds = reinterpret_cast<float *>(lds_raw);  // staged atoms, exact float32
     __shared__ float sh_rowM[4][MAX_RB_EXPF];
@@ -1388,7 +1405,7 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     const int n = col0 + threadIdx.x;
     const bool valid = n < N_tau;
     const int col_last = min(col0 + TILE_COLS, N_tau) - 1;
-    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_atoms, r_top, row0, col0,
+    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_ok, r_top, row0, col0,
                                   col_last);
     const ExpCol ec = exp_col(td, valid ? n : col0);
     double E[NC];
@@ -1902,9 +1919,8 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
                     p.seg_items.push_back(ItemDev{(int32_t)t, (int32_t)sg, (int32_t)cbk, 0});
             for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
                 p.col_items.push_back(ItemDev{(int32_t)t, 0, (int32_t)cbk, 0});
-            const int need = expf_stage_atoms(td.a, rb + 1, td.Ls, TILE_COLS);
-            if (need <= EXPF_LDS_CAP)  // planes at a fixed stride: the full capacity
-                p.max_lds_expf = (int)(sizeof(float) * NC * EXPF_LDS_CAP);
+            if (expf_fits_lds(td.a, rb + 1, td.Ls, TILE_COLS))  // fixed plane strides
+                p.max_lds_expf = EXPF_LDS_BYTES;
         }
     } else {
         td.prefix_off = p.prefix_doubles;
@@ -2037,7 +2053,7 @@ void launch_maps(tcw_plan &p, float *F, hipStream_t st) {
         hipLaunchKernelGGL((exp_fast_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG),
                            p.max_lds_expf, st, tds, items, p.mode_begin[MODE_EXP_FAST], merged,
                            (const double *)p.d_etab.p, F, ipart, rowpart, colpart, flags,
-                           p.max_lds_expf / (int)(sizeof(float) * NC));
+                           p.max_lds_expf > 0 ? 1 : 0);
     if ((n = p.mode_begin[MODE_EXP_DIRECT + 1] - p.mode_begin[MODE_EXP_DIRECT]) > 0)
         hipLaunchKernelGGL((exp_direct_kernel<BTSG, STRICT>), dim3((unsigned)n), dim3(WG), 0,
                            st, tds, items, p.mode_begin[MODE_EXP_DIRECT], merged, F, ipart,
@@ -2076,7 +2092,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
         hipLaunchKernelGGL(exp_seg_kernel, dim3((unsigned)p.seg_items.size()), dim3(WG),
                            p.max_lds_expf, st, (const TmplDev *)p.d_tmpl.p,
                            (const ItemDev *)p.d_seg_items.p, (const double *)p.d_merged.p,
-                           (double *)p.d_etab.p, p.max_lds_expf / (int)(sizeof(float) * NC));
+                           (double *)p.d_etab.p, p.max_lds_expf > 0 ? 1 : 0);
     if (!p.col_items.empty())
         hipLaunchKernelGGL(exp_scan_kernel, dim3((unsigned)p.col_items.size()), dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
