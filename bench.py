@@ -400,6 +400,31 @@ def run_refine(args):
     prob.close()
 
 
+def measured_copy_gbps(device, reps=5):
+    """Device-to-device copy bandwidth (1 GiB read + 1 GiB written per copy),
+    HIP events on the current stream, best of reps -- the achievable HBM
+    figure the map kernels are also compared with."""
+    import torch
+
+    src = torch.empty(2**28, dtype=torch.float32, device=device)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    stream = torch.cuda.current_stream()
+    best = None
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        dst.copy_(src)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * 2**30 / (best / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -512,6 +537,8 @@ def main():
     log(f"stage/maps/finalize ms: {stage_ms.round(4).tolist()}")
 
     atoms_binned = int(b.stats()["numAtoms"].sum())
+    copy_gbps = measured_copy_gbps(b.device)
+    log(f"1 GiB device copy: {copy_gbps:.0f} GB/s")
     parity = parity_sample(b, batch, args.check_templates) if batch is not None else None
     if parity:
         log(f"parity: {parity}")
@@ -580,6 +607,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "copy_GBps": copy_gbps,
+            "frac_of_copy": achieved / copy_gbps,
             "traffic": traffic,
             "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": map_bytes,
