@@ -242,6 +242,40 @@ def test_batched_rect_long_templates(hip_ready):
     b.close()
 
 
+@pytest.mark.parametrize("amp", [0.0, 3.0])
+@pytest.mark.parametrize("wtype", [TRANSIENT_RECTANGULAR, TRANSIENT_EXPONENTIAL])
+def test_reference_test_configuration(hip_ready, wtype, amp):
+    """The set-up of tests/test_tcw_fstat_map_funcs.py:15-127: one day of 1800 s
+    SFTs from GPS 700000000, one detector, t0 over [start, start + day - 2 Tsft],
+    tau over [2 Tsft, day] in Tsft steps, a transient with tau = 0.25 day at
+    t0 = start + 0.25 day (none for amp = 0).  The reference compares its map
+    with lalpulsar's search of the same data; here the whole map, ML point,
+    lnBtSG and posterior modes are compared with the oracle.  (With one
+    detector, windows of 2-3 SFTs sit just below the cond = 1e4 guard and
+    reach F ~ 1e4 in noise, so the ML point is not where the injection is;
+    that is the formula, and the oracle agrees.)"""
+    Tstart, Tsft, day = 700000000, 1800, 86400
+    rng = np.random.default_rng(101 + int(amp))
+    t = Tstart + Tsft * np.arange(day // Tsft, dtype=np.int64)
+    t_on, tau_inj = Tstart + day // 4, day // 4
+    if amp == 0:
+        sig = np.zeros(t.size, np.complex128)
+    elif wtype == TRANSIENT_RECTANGULAR:
+        sig = synth.rect_signal(t.astype(np.float64), t_on, tau_inj, 0.7, amp=amp)
+    else:
+        sig = synth.exp_signal(t.astype(np.float64), t_on, tau_inj, 0.7, amp=amp)
+    cols = synth.detector_atoms(rng, t.astype(np.float64), 0.2, 1.3, sig)
+    multi = synth.multi_from_detectors([(t, cols)], Tsft=Tsft)
+    win = transientWindowRange_t(wtype, Tstart, day - 2 * Tsft, Tsft, 2 * Tsft,
+                                 day - 2 * Tsft, Tsft)
+    fm, om, _ = _check_full(multi, win)
+    assert not np.isnan(fm.maxF) and not np.isnan(fm.lnBtSG)
+    if wtype == TRANSIENT_RECTANGULAR:
+        # first t0, longest tau: the all-data F (tests/test_tcw_fstat_map_funcs.py:124-127)
+        full = oracle_for(multi, transientWindowRange_t(TRANSIENT_NONE, 0, 0, 0, 0, 0, 0))
+        assert fm.F_mn[0, -1] == pytest.approx(full.F_mn[0, 0], rel=1e-4)
+
+
 def test_lal_strict_single_atom(hip_ready):
     """lal-strict mode raises where lalpulsar raises XLAL_EDOM; default mode
     returns the F = 2 fallback there."""
