@@ -1,7 +1,7 @@
 """FP64 CPU oracle of the transient F-statistic map -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import
-this package, and only as the checker.  The product (``pyfstat_amd``) never
+this package, and only to check results or to time them.  The product (``pyfstat_amd``) never
 imports, links or runs anything here.
 
 Two independent restatements of the reference algorithm live here:

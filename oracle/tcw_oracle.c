@@ -1,7 +1,7 @@
 /*
  * TEST INFRASTRUCTURE ONLY -- never linked into, loaded by or called from the
  * product library.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
- * baseline leg may use this file, and only as the checker.
+ * baseline leg may use this file, and only to check results or to time them.
  *
  * FP64 CPU restatement of the transient-CW F-statistic map of lalpulsar's
  * XLALComputeTransientFstatMap() (LALSuite, TransientCW_utils.c; not vendored in

@@ -4,7 +4,7 @@ import numpy as np
 
 import oracle
 
-# north-star tolerance on F (and 2F): max relative error vs the FP64 oracle
+# tolerance on F (and 2F): max relative error vs the FP64 oracle
 F_RTOL = 1e-4
 # cells whose condition number sits within this relative distance of the 1e4
 # guard may legitimately flip between the formula and the F = 2 fallback
