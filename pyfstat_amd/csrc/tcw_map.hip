@@ -1539,6 +1539,11 @@ __global__ __launch_bounds__(NT) void finalize_kernel(
     const float *__restrict__ rowpart, const float *__restrict__ colpart,
     const int *__restrict__ flags, tcw_stats *__restrict__ stats) {
     __shared__ float sh_scale[FIN_SCALE_CAP];
+    // the wide variant (few, large maps: many partials per row and column)
+    // unrolls the partial sums by 8 so their independent loads issue
+    // together; the narrow one (many templates) is faster as the compiler
+    // schedules it by itself
+    constexpr bool FIN_UNROLL = NT >= 1024;
     const int t = blockIdx.x;
     const TmplDev td = tds[t];
     const int64_t nitems = (int64_t)td.n_rowblk * td.n_colblk;
@@ -1577,9 +1582,16 @@ __global__ __launch_bounds__(NT) void finalize_kernel(
         for (int m = threadIdx.x; m < (int)td.N_t0; m += NT) {
             const int rb = m / td.rb_rows;
             const float *rp = rowpart + td.rowpart_off + m;  // [colblk][N_t0]
+            auto term = [&](uint32_t cb) {
+                return (double)(rp[(int64_t)cb * td.N_t0] * scale((int64_t)rb * td.n_colblk + cb));
+            };
             double R = 0.0;
-            for (uint32_t cb = 0; cb < td.n_colblk; cb++)
-                R += (double)(rp[(int64_t)cb * td.N_t0] * scale((int64_t)rb * td.n_colblk + cb));
+            if constexpr (FIN_UNROLL) {
+#pragma unroll 8
+                for (uint32_t cb = 0; cb < td.n_colblk; cb++) R += term(cb);
+            } else {
+                for (uint32_t cb = 0; cb < td.n_colblk; cb++) R += term(cb);
+            }
             if (R > rv || (R == rv && m < ri)) {
                 rv = R;
                 ri = m;
@@ -1592,9 +1604,16 @@ __global__ __launch_bounds__(NT) void finalize_kernel(
         for (int n = threadIdx.x; n < (int)td.N_tau; n += NT) {
             const int cb = n / TILE_COLS;
             const float *cp = colpart + td.colpart_off + n;  // [rowblk][N_tau]
+            auto term = [&](uint32_t rb) {
+                return (double)(cp[(int64_t)rb * td.N_tau] * scale((int64_t)rb * td.n_colblk + cb));
+            };
             double Cs = 0.0;
-            for (uint32_t rb = 0; rb < td.n_rowblk; rb++)
-                Cs += (double)(cp[(int64_t)rb * td.N_tau] * scale((int64_t)rb * td.n_colblk + cb));
+            if constexpr (FIN_UNROLL) {
+#pragma unroll 8
+                for (uint32_t rb = 0; rb < td.n_rowblk; rb++) Cs += term(rb);
+            } else {
+                for (uint32_t rb = 0; rb < td.n_rowblk; rb++) Cs += term(rb);
+            }
             if (Cs > cv || (Cs == cv && n < ci)) {
                 cv = Cs;
                 ci = n;
