@@ -279,15 +279,17 @@ __global__ __launch_bounds__(SR_WG) void stage_kernel(
 
 // Exponential-window templates: the merged atoms as the FP64 table [N][8]
 // (7 columns + padding), the same float32 detector-order sums as
-// stage_kernel.  A workgroup owns SX_BINS_BLOCK consecutive bins of one
-// template: per detector, two binary searches bound the block's atoms, which
-// are copied into LDS in chunks with coalesced loads; each thread then sums its
-// SX_BINS bins from LDS (atoms in order, detectors in order).  Grid: (bin
-// blocks, exp templates), y-strided.
-constexpr int SX_BINS = 4;
-constexpr int SX_BINS_BLOCK = 256 * SX_BINS;
+// stage_kernel.  A workgroup owns 256 * SXB consecutive bins of one template:
+// per detector, two binary searches bound the block's atoms, which are copied
+// into LDS in chunks with coalesced loads; each thread then sums its SXB bins
+// from LDS (atoms in order, detectors in order).  Grid: (bin blocks, exp
+// templates), y-strided.  SXB = 4 (1024-bin blocks) for batches, 1 (256-bin
+// blocks) when 1024-bin blocks would give fewer than SX_NARROW_BELOW
+// workgroups (one large map would otherwise occupy a handful of CUs).
+constexpr int SX_NARROW_BELOW = 256;
 constexpr int SX_CHUNK = 1024;  // atoms staged per pass
 
+template <int SXB>
 __global__ __launch_bounds__(256) void stage_exp_kernel(
     const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ ts,
@@ -298,6 +300,8 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
     const int tid = threadIdx.x;
     for (int64_t e = blockIdx.y; e < n_exp; e += gridDim.y) {
         const TmplDev td = tds[exp_tmpls[e]];
+        constexpr int SX_BINS = SXB;
+        constexpr int SX_BINS_BLOCK = 256 * SXB;
         const uint32_t B0 = blockIdx.x * (uint32_t)SX_BINS_BLOCK;
         if (B0 >= td.N) continue;  // block-uniform
         const uint32_t B1 = min(td.N, B0 + (uint32_t)SX_BINS_BLOCK);
@@ -1304,12 +1308,18 @@ __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict_
 }
 
 // Per tau column: E at every chunk top.  Work items (t, -, cb).
-__global__ __launch_bounds__(WG) void exp_scan_kernel(const TmplDev *__restrict__ tds,
-                                                      const ItemDev *__restrict__ items,
-                                                      double *__restrict__ etab) {
-    const ItemDev it = items[block_slot()];
+// One 64-thread workgroup per quarter of a column block: the walk is a chain
+// of memory operations per column, and small batches (one large map) have few
+// column blocks, so narrow workgroups spread them over 4x the CUs.
+constexpr int SCAN_WG = 64;
+constexpr int SCAN_SPLIT = TILE_COLS / SCAN_WG;
+
+__global__ __launch_bounds__(SCAN_WG) void exp_scan_kernel(const TmplDev *__restrict__ tds,
+                                                           const ItemDev *__restrict__ items,
+                                                           double *__restrict__ etab) {
+    const ItemDev it = items[blockIdx.x / SCAN_SPLIT];
     const TmplDev td = tds[it.t];
-    const int n = it.cb * TILE_COLS + threadIdx.x;
+    const int n = it.cb * TILE_COLS + (int)(blockIdx.x % SCAN_SPLIT) * SCAN_WG + threadIdx.x;
     if (n >= (int)td.N_tau) return;
     const double tau = (double)td.tau + (double)n * (double)td.dtau;
     const double steps = (double)td.a * td.rb_rows;
@@ -2095,13 +2105,21 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(SR_WG), 0, st,
                        (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
                        cstride, (double *)p.d_prefix.p);
-    if (!p.exp_tmpls.empty())
-        hipLaunchKernelGGL(stage_exp_kernel,
-                           dim3((unsigned)((p.exp_max_N + SX_BINS_BLOCK - 1) / SX_BINS_BLOCK),
-                                (unsigned)std::min<size_t>(p.exp_tmpls.size(), 65535)),
-                           dim3(256), 0, st, (const TmplDev *)p.d_tmpl.p,
-                           (const int32_t *)p.d_exp_tmpls.p, (int64_t)p.exp_tmpls.size(),
-                           (const SegDev *)p.d_segs.p, ts, cols, cstride, (double *)p.d_merged.p);
+    if (!p.exp_tmpls.empty()) {
+        const int64_t n_exp = (int64_t)p.exp_tmpls.size();
+        const int64_t wide_blocks = (p.exp_max_N + 1023) / 1024;
+        const unsigned gy = (unsigned)std::min<int64_t>(n_exp, 65535);
+        if (n_exp > 0 && wide_blocks * n_exp < SX_NARROW_BELOW)
+            hipLaunchKernelGGL(stage_exp_kernel<1>, dim3((unsigned)((p.exp_max_N + 255) / 256), gy),
+                               dim3(256), 0, st, (const TmplDev *)p.d_tmpl.p,
+                               (const int32_t *)p.d_exp_tmpls.p, n_exp, (const SegDev *)p.d_segs.p,
+                               ts, cols, cstride, (double *)p.d_merged.p);
+        else if (n_exp > 0)
+            hipLaunchKernelGGL(stage_exp_kernel<4>, dim3((unsigned)wide_blocks, gy), dim3(256), 0,
+                               st, (const TmplDev *)p.d_tmpl.p, (const int32_t *)p.d_exp_tmpls.p,
+                               n_exp, (const SegDev *)p.d_segs.p, ts, cols, cstride,
+                               (double *)p.d_merged.p);
+    }
     if (!p.col_items.empty())
         hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)(p.col_items.size() * INIT_PARTS)),
                            dim3(WG), 0, st,
@@ -2113,7 +2131,8 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
                            (const ItemDev *)p.d_seg_items.p, (const double *)p.d_merged.p,
                            (double *)p.d_etab.p, p.max_lds_expf > 0 ? 1 : 0);
     if (!p.col_items.empty())
-        hipLaunchKernelGGL(exp_scan_kernel, dim3((unsigned)p.col_items.size()), dim3(WG), 0, st,
+        hipLaunchKernelGGL(exp_scan_kernel, dim3((unsigned)(p.col_items.size() * SCAN_SPLIT)),
+                           dim3(SCAN_WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
                            (double *)p.d_etab.p);
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[1], st));
