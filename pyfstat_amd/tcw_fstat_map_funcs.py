@@ -449,6 +449,8 @@ def register_into_pyfstat():
     ref = _optional_import("pyfstat.tcw_fstat_map_funcs")
     if ref is None:
         return False
+    if getattr(ref, "_hip_registered", False):  # idempotent
+        return True
     ref.fstatmap_versions["hip"] = fstatmap_versions["hip"]
     orig_features = ref._get_transient_fstat_map_features
     orig_init = ref.init_transient_fstat_map_features
@@ -465,6 +467,7 @@ def register_into_pyfstat():
 
     ref._get_transient_fstat_map_features = _features
     ref.init_transient_fstat_map_features = _init
+    ref._hip_registered = True
     return True
 
 
