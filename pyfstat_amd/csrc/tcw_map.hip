@@ -45,6 +45,7 @@ constexpr double COND_MAX = 1e4;  // rect.cu:104
 // cond < K  <=>  T^2 < Q*D  with T = A+B, D = AB-C^2 > 0, Q = (K+1)^2/K, because
 // cond = (T+s)/(T-s), s^2 = T^2-4D, and (1+cond)^2/cond = T^2/D grows with cond.
 constexpr double QCOND = (COND_MAX + 1.0) * (COND_MAX + 1.0) / COND_MAX;
+constexpr float QCONDF = (float)QCOND;
 
 enum Mode : int32_t {
     MODE_RECT_FAST = 0,     // regular grid: dt0 == dtau == a*TAtom, no wrap-around
@@ -390,13 +391,16 @@ __device__ __forceinline__ float cell_F(double A, double B, double C, double far
     const double q = fma(fbr, fbr, fbi * fbi);
     const double r = fma(far, fbr, fai * fbi);
     const double num = fma(B, p, fma(A, q, (-2.0 * C) * r));
-    // A, B >= 0 (sums of squares), so T <= 0 only for an empty window, where
-    // D = 0 and the test below fails as well
-    const bool ok = T * T < QCOND * D;
     // the map is float32: the quotient of the FP64 numerator and determinant
     // (each rounded to float) through v_rcp_f32 is within ~3e-7 relative, and
-    // v_rcp_f64 issues at ~1/3.5 the rate of an FP64 FMA
-    const float F = (float)num * __builtin_amdgcn_rcpf((float)D);
+    // v_rcp_f64 issues at ~1/3.5 the rate of an FP64 FMA.  The condition test
+    // runs on the same float D (and float T): it differs from an FP64 test only
+    // for windows within ~1e-7 of cond = 1e4.  A, B >= 0 (sums of squares), so
+    // T <= 0 only for an empty window, where D = 0 and the test fails as well.
+    const float Df = (float)D;
+    const float Tf = (float)T;
+    const bool ok = Tf * Tf < QCONDF * Df;
+    const float F = (float)num * __builtin_amdgcn_rcpf(Df);
     return ok ? F : 2.0f;
 }
 
