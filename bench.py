@@ -68,7 +68,7 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # What bounds the map kernel instead of HBM (rocprofv3 counters of the same
 # kernel, summarised in the profile files named here).
 LIMITER = {
-    "rect_batched": "FP64 VALU issue: ~45 VALU instructions per cell, SQ_ACTIVE_INST_VALU ~96% of "
+    "rect_batched": "FP64 VALU issue: ~46 VALU instructions per cell, SQ_ACTIVE_INST_VALU ~96-99% of "
                     "SIMD cycles at the engine clock the card holds under this load (GRBM_GUI_ACTIVE: "
                     "~1.8-1.9 GHz, not 2.4); profiles/rect_fast_kernel.rect_batched.md",
     "exp_shared": "FP64 VALU issue (recurrence step + F per cell); "
