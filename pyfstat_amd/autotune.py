@@ -116,10 +116,16 @@ def tune(b, reps=3, candidates=None, use_cache=True, device_name=None, log=None)
             continue
         key = shape_class(b, mode, device_name)
         if use_cache and key in cache:
-            set_rows(mode, cache[key]["rows"])
-            result[mode] = dict(cache[key], cached=True)
-            changed = True
-            continue
+            try:
+                set_rows(mode, cache[key]["rows"])
+            except (ValueError, KeyError, TypeError):
+                # a value this library no longer accepts (its limits changed
+                # under the same version string): tune again
+                del cache[key]
+            else:
+                result[mode] = dict(cache[key], cached=True)
+                changed = True
+                continue
         tried = {}
         for rows in cands[mode]:
             set_rows(mode, rows)
