@@ -235,8 +235,15 @@ def hip_compute_transient_fstat_maps(multiFstatAtoms_list, windowRanges, BtSG=Fa
     """Batched counterpart of hip_compute_transient_fstat_map: one
     pyTransientFstatMap per (atoms, windowRange) pair, all computed in one
     device pass."""
+    _hiplib.load()  # fail loudly without the library, also for an empty batch
+    multiFstatAtoms_list = list(multiFstatAtoms_list)
     if not isinstance(windowRanges, (list, tuple)):
         windowRanges = [windowRanges] * len(multiFstatAtoms_list)
+    if len(windowRanges) != len(multiFstatAtoms_list):
+        raise ValueError(f"{len(multiFstatAtoms_list)} atom sets but {len(windowRanges)} "
+                         "window ranges")
+    if not multiFstatAtoms_list:
+        return []
     b = BatchedTransientMaps.from_multi_atoms(multiFstatAtoms_list, windowRanges, BtSG=BtSG,
                                               lal_strict=lal_strict)
     try:

@@ -84,3 +84,19 @@ def test_random_batch(hip_ready):
         compare_maps(fm.F_mn, om)
         assert abs(fm.lnBtSG - om.lnBtSG) <= lnb_tol(om.maxF)
         _posterior_ok(om, win, fm.t0_MP, fm.tau_MP)
+
+
+@pytest.mark.parametrize("n_atoms", [1, 2, 3])
+@pytest.mark.parametrize("wtype", [TRANSIENT_RECTANGULAR, TRANSIENT_EXPONENTIAL])
+def test_smallest_templates(hip_ready, n_atoms, wtype):
+    """Templates of 1-3 atoms (one merged bin per atom): most windows hold a
+    single atom (D = 0 exactly, F = 2 fallback) or none."""
+    rng = np.random.default_rng(7 + n_atoms)
+    t = (synth.T_START + TA * np.arange(n_atoms)).astype(np.int64)
+    cols = synth.detector_atoms(rng, t.astype(np.float64), 0.4, 1.7,
+                                np.zeros(n_atoms, np.complex128))
+    multi = synth.multi_from_detectors([(t, cols)], Tsft=TA)
+    win = transientWindowRange_t(wtype, synth.T_START - TA, 4 * TA, TA, TA, 4 * TA, TA)
+    _check_full(multi, win)
+    if wtype == TRANSIENT_RECTANGULAR:
+        _check_full(multi, win, force_generic=True)

@@ -191,11 +191,12 @@ def test_autograd_backward_vs_oracle(small):
     assert np.all(np.abs(t0.grad.cpu().numpy()) <= 1e-8 * np.abs(gF * F) / prob.tau_init.cpu().numpy())
     want = np.zeros(merged.cols.shape)
     cols_h = merged.cols.cpu().numpy()
+    t0_h, tau_h = t0.detach().cpu().numpy(), tau.detach().cpu().numpy()
     for i in range(prob.n):
         t = int(prob.cand_tmpl[i])
         a, e = int(merged.m_off[t]), int(merged.m_off[t + 1])
         want[:, a:e] += rr.atom_grad((7, e - a), float(merged.t_data[t]), TA, int(prob.k_lo[i]),
-                                     int(prob.k_hi[i]), float(t0[i]), float(tau[i]), g[i], gF[i])
+                                     int(prob.k_hi[i]), float(t0_h[i]), float(tau_h[i]), g[i], gF[i])
     got = cols.grad.cpu().numpy()
     scale = np.abs(want).max()
     assert np.abs(got - want).max() <= 2e-6 * scale
