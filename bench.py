@@ -265,6 +265,10 @@ def run_refine(args):
     torch.cuda.synchronize()
     log(f"ready in {time.perf_counter() - t_gen:.1f} s: {prob.n} candidates, mean support "
         f"{L.mean():.1f} bins")
+    edges = [0, 8, 32, 128, 512, int(L.max())]
+    classes = {f"{lo + 1}-{hi}": [int(((L > lo) & (L <= hi)).sum()), int(L[(L > lo) & (L <= hi)].sum())]
+               for lo, hi in zip(edges[:-1], edges[1:]) if hi > lo}
+    log(f"support lengths (bins): candidates, total bins per class {classes}")
     n_steps = args.refine_steps
     for _ in range(args.warmup):
         prob.ascent(merged.cols, tau_lo=lo, tau_hi=hi, n_steps=n_steps)
@@ -371,6 +375,7 @@ def run_refine(args):
             "candidates": prob.n,
             "ascent_steps": n_steps,
             "mean_support_bins": float(L.mean()),
+            "support_classes": classes,
         },
         "roofline": {
             "bound": "valu",

@@ -34,12 +34,20 @@ using tcwc::set_device;
 namespace {
 
 constexpr int NC = TCW_NCOL;
-constexpr int RWG = 256;             // 4 waves
-constexpr int GROUP = 16;            // lanes per candidate
-constexpr int GROUPS_PER_WAVE = 64 / GROUP;
-constexpr int CAND_PER_ITEM = (RWG / 64) * GROUPS_PER_WAVE;  // 16: 4 per wave
+constexpr int RWG = 256;             // threads per block of the register kernels
+constexpr int CAND_PER_ITEM = 16;    // candidates per LDS item (one block)
+constexpr int LONG_G = 16;           // lanes per candidate in the LDS kernels
 constexpr int LDS_ATOMS_CAP = 2176;  // staged support bins: 7*4*2176 = 60928 B
 constexpr int BWD_BINS = 256;        // bins per backward block
+// Short supports are held in registers, RB bins per lane, loaded once per
+// launch: up to RB bins on one lane per candidate, up to 4*RB bins on four
+// lanes (two DPP steps instead of four), longer ones on the 16-lane groups
+// reading the item's support union from LDS.  Grid maxima mostly sit at short
+// windows (median support ~6 bins), where a 16-lane group idles in the bin
+// loop and repeats the per-evaluation exp / division / 2x2 algebra 16 times.
+constexpr int RB = 8;
+constexpr int SHORT1_MAX = RB;
+constexpr int SHORT4_MAX = 4 * RB;
 constexpr double COND_MAX = 1e4;
 constexpr double QCOND = (COND_MAX + 1.0) * (COND_MAX + 1.0) / COND_MAX;  // cond < K <=> T^2 < Q D
 constexpr double LN70 = 4.248495242049359;  // log(70)
@@ -53,7 +61,7 @@ struct RTmpl {
 
 struct RItem {
     int64_t t;      // template
-    int64_t cbeg;   // first candidate (caller order, grouped by template)
+    int64_t cbeg;   // first slot in the plan's candidate order (long supports first)
     int32_t ncand;  // candidates in the item
     int32_t u_lo;   // first staged bin (relative to the template)
     int32_t u_len;  // staged bins
@@ -100,21 +108,23 @@ __device__ __forceinline__ void group_step(double (&s)[N]) {
     for (int i = 0; i < N; i++) s[i] += t[i];
 }
 
-// Sum each value over the 16 lanes of the group; every lane gets the same
-// totals.
-template <int N>
-__device__ __forceinline__ void group_sum16(double (&s)[N]) {
-    group_step<DPP_ROW_MIRROR>(s);
-    group_step<DPP_ROW_HALF_MIRROR>(s);
-    group_step<DPP_QUAD_XOR2>(s);
-    group_step<DPP_QUAD_XOR1>(s);
+// Sum each value over the G lanes of an aligned group (G = 1, 4, 8, 16);
+// every lane gets the same totals.
+template <int G, int N>
+__device__ __forceinline__ void group_sum(double (&s)[N]) {
+    static_assert(G == 1 || G == 4 || G == 8 || G == 16, "group size");
+    if constexpr (G == 16) group_step<DPP_ROW_MIRROR>(s);
+    if constexpr (G >= 8) group_step<DPP_ROW_HALF_MIRROR>(s);
+    if constexpr (G >= 4) {
+        group_step<DPP_QUAD_XOR2>(s);
+        group_step<DPP_QUAD_XOR1>(s);
+    }
 }
 
-// Window sums of one candidate per 16-lane group (4 candidates per wave):
-// s[0..6] = S_c, s[7..13] = U_c.  Atom (c, k) of the template is
-// src[c*sstride + k - sbase] (src in LDS or in global memory); an empty
-// support (kb > ke) gives zeros.
-template <typename P>
+// Window sums of one candidate per G-lane group: s[0..6] = S_c, s[7..13] =
+// U_c.  Atom (c, k) of the template is src[c*sstride + k - sbase] (src in
+// LDS or in global memory); an empty support (kb > ke) gives zeros.
+template <int G, typename P>
 __device__ __forceinline__ void group_sums(P src, int64_t sstride, int sbase, int kb, int ke,
                                            double t_data, double TA, double t0, double inv_tau,
                                            int gl, double (&s)[2 * NC]) {
@@ -123,9 +133,9 @@ __device__ __forceinline__ void group_sums(P src, int64_t sstride, int sbase, in
     const int k0 = kb + gl;
     double u = fma((double)k0, TA, t_data - t0);  // t_k - t0
     double w = exp(-u * inv_tau);
-    const double rg = exp(-(double)GROUP * TA * inv_tau);
-    const double du = (double)GROUP * TA;
-    for (int k = k0; k <= ke; k += GROUP) {
+    const double rg = exp(-(double)G * TA * inv_tau);
+    const double du = (double)G * TA;
+    for (int k = k0; k <= ke; k += G) {
         const int j = k - sbase;
         const double w2 = w * w;
 #pragma unroll
@@ -137,7 +147,7 @@ __device__ __forceinline__ void group_sums(P src, int64_t sstride, int sbase, in
         w *= rg;
         u += du;
     }
-    group_sum16(s);
+    group_sum<G>(s);
 }
 
 // F and its derivatives from the window sums (group-uniform).
@@ -173,6 +183,17 @@ __device__ __forceinline__ Eval eval_from_sums(const double (&s)[2 * NC], double
     return e;
 }
 
+// Entry q of a candidate's saved row: dF/dt0, dF/dtau, dF/dS_c (7), F.
+__device__ __forceinline__ double saved_value(const Eval &e, int q) {
+    double v = e.F;  // q == 9
+    if (q == 0) v = e.dFdt0;
+    if (q == 1) v = e.dFdtau;
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+        if (q == 2 + c) v = e.g[c];
+    return v;
+}
+
 // The atoms of an item: its LDS copy of the support union, or the template's
 // bins in global memory when the union does not fit.
 struct Src {
@@ -182,19 +203,22 @@ struct Src {
     bool in_lds;
 };
 
+template <int G>
 __device__ __forceinline__ Eval eval_group(const Src &sr, const float *lds, int kb, int ke,
                                            double t_data, double TA, double t0, double tau,
                                            int gl) {
     const double inv_tau = 1.0 / tau;
     double s[2 * NC];
     if (sr.in_lds)  // block-uniform
-        group_sums(lds, sr.lstride, sr.lbase, kb, ke, t_data, TA, t0, inv_tau, gl, s);
+        group_sums<G>(lds, sr.lstride, sr.lbase, kb, ke, t_data, TA, t0, inv_tau, gl, s);
     else
-        group_sums(sr.g, sr.gstride, 0, kb, ke, t_data, TA, t0, inv_tau, gl, s);
+        group_sums<G>(sr.g, sr.gstride, 0, kb, ke, t_data, TA, t0, inv_tau, gl, s);
     return eval_from_sums(s, inv_tau);
 }
 
-// Stage the item's support union (component-major) into LDS when it fits.
+// Stage the item's support union (component-major) into LDS when it fits
+// (NT threads per block).
+template <int NT>
 __device__ __forceinline__ Src stage_support(const RItem &it, const RTmpl &tp,
                                              const float *__restrict__ cols, int64_t cstride,
                                              float *lds) {
@@ -207,64 +231,61 @@ __device__ __forceinline__ Src stage_support(const RItem &it, const RTmpl &tp,
     if (sr.in_lds) {
         const float *g = cols + tp.m_off + it.u_lo;
         for (int c = 0; c < NC; c++)
-            for (int k = threadIdx.x; k < it.u_len; k += RWG) lds[c * it.u_len + k] = g[c * cstride + k];
+            for (int k = threadIdx.x; k < it.u_len; k += NT) lds[c * it.u_len + k] = g[c * cstride + k];
         __syncthreads();
     }
     return sr;
 }
 
-// Candidate slot of a lane: group (lane / 16) of wave w handles item
-// candidate j = w*4 + group; lanes without a candidate get an empty support.
+// Candidate slot of a lane: G-lane group j = thread / G handles item
+// candidate order[cbeg + j]; lanes without a candidate get an empty support.
 struct Slot {
     int64_t ci;
     bool has;
     int kb, ke;
 };
 
-__device__ __forceinline__ Slot lane_slot(const RItem &it, const int32_t *k_lo,
-                                          const int32_t *k_hi) {
-    const int lane = threadIdx.x & 63;
-    const int j = (threadIdx.x >> 6) * GROUPS_PER_WAVE + (lane >> 4);
+template <int G>
+__device__ __forceinline__ Slot lane_slot(const RItem &it, const int32_t *order,
+                                          const int32_t *k_lo, const int32_t *k_hi) {
+    const int j = (int)threadIdx.x / G;
     Slot sl;
     sl.has = j < it.ncand;
-    sl.ci = it.cbeg + (sl.has ? j : 0);
+    sl.ci = order[it.cbeg + (sl.has ? j : 0)];
     sl.kb = sl.has ? k_lo[sl.ci] : 1;
     sl.ke = sl.has ? k_hi[sl.ci] : 0;
     return sl;
 }
 
-__global__ __launch_bounds__(RWG) void refine_forward_kernel(
-    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ k_lo,
+template <int G>
+__global__ __launch_bounds__(CAND_PER_ITEM * G) void refine_forward_kernel(
+    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ order,
+    const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
     const double *__restrict__ t0v, const double *__restrict__ tauv, double *__restrict__ twoF,
     double *__restrict__ lnB, double *__restrict__ saved) {
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
     const RTmpl tp = tm[it.t];
-    const Src sr = stage_support(it, tp, cols, cstride, lds);
-    const int lane = threadIdx.x & 63, gl = lane & (GROUP - 1);
-    const Slot sl = lane_slot(it, k_lo, k_hi);
+    const Src sr = stage_support<CAND_PER_ITEM * G>(it, tp, cols, cstride, lds);
+    const int gl = (int)threadIdx.x % G;
+    const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
     const double tau = sl.has ? tauv[sl.ci] : 1.0;
-    const Eval e = eval_group(sr, lds, sl.kb, sl.ke, (double)tp.t_data, TA, t0, tau, gl);
+    const Eval e = eval_group<G>(sr, lds, sl.kb, sl.ke, (double)tp.t_data, TA, t0, tau, gl);
     if (!sl.has) return;
     if (gl == 0) {
         twoF[sl.ci] = 2.0 * e.F;
         if (lnB) lnB[sl.ci] = LN70 + e.F;
     }
-    if (saved && gl < NSAVED) {
-        double x = e.F;
-        if (gl == 0) x = e.dFdt0;
-        if (gl == 1) x = e.dFdtau;
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-            if (gl == 2 + c) x = e.g[c];
-        saved[sl.ci * NSAVED + gl] = x;
-    }
+    if (saved)
+        for (int q = gl; q < NSAVED; q += G) saved[sl.ci * NSAVED + q] = saved_value(e, q);
 }
 
-__global__ __launch_bounds__(RWG) void refine_ascent_kernel(
-    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ k_lo,
+template <int G>
+__global__ __launch_bounds__(CAND_PER_ITEM * G) void refine_ascent_kernel(
+    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ order,
+    const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
     const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
     const double *__restrict__ tau_hi, int n_steps, double step0, double *__restrict__ twoF,
@@ -272,9 +293,9 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
     const RTmpl tp = tm[it.t];
-    const Src sr = stage_support(it, tp, cols, cstride, lds);
-    const int lane = threadIdx.x & 63, gl = lane & (GROUP - 1);
-    const Slot sl = lane_slot(it, k_lo, k_hi);
+    const Src sr = stage_support<CAND_PER_ITEM * G>(it, tp, cols, cstride, lds);
+    const int gl = (int)threadIdx.x % G;
+    const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
     const double tau_in = sl.has ? tauv[sl.ci] : 1.0;
     const bool live = sl.has && tau_in > 0.0;
@@ -282,7 +303,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     const double lo = (live && tau_lo) ? log(tau_lo[sl.ci]) : -INFINITY;
     const double hi = (live && tau_hi) ? log(tau_hi[sl.ci]) : INFINITY;
     const int kb = live ? sl.kb : 1, ke = live ? sl.ke : 0;
-    Eval e = eval_group(sr, lds, kb, ke, (double)tp.t_data, TA, t0, tau, gl);
+    Eval e = eval_group<G>(sr, lds, kb, ke, (double)tp.t_data, TA, t0, tau, gl);
     double s = log(tau);
     double g = tau * e.dFdtau;  // dF/dlog(tau)
     double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
@@ -293,7 +314,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
         const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
         const bool moved = sn != s;
         const double taun = moved ? exp(sn) : tau;
-        const Eval en = eval_group(sr, lds, kb, ke, (double)tp.t_data, TA, t0, taun, gl);
+        const Eval en = eval_group<G>(sr, lds, kb, ke, (double)tp.t_data, TA, t0, taun, gl);
         const bool accept = moved && en.F > e.F;
         if (accept) {
             s = sn;
@@ -308,6 +329,163 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
         tauv[sl.ci] = live ? tau : tau_in;
         twoF[sl.ci] = live ? 2.0 * e.F : NAN;
         if (n_accept) n_accept[sl.ci] = acc;
+    }
+}
+
+// ---- short supports in registers: G lanes per candidate (G = 1 or 4) ----
+// Lane gl of a group holds bins kb + gl + G*j, j < RB; positions past ke hold
+// zeros and add nothing to the sums.
+template <int G>
+struct RegSlot {
+    int64_t ci;
+    bool has;
+    int kb, ke, gl;
+    int nw;  // bins per lane to walk: the maximum over the wave (wave-uniform)
+    float x[RB][NC];
+};
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Called by every lane of the block (the wave maximum needs them all).
+template <int G>
+__device__ __forceinline__ void reg_slot(RegSlot<G> &r, const RTmpl *__restrict__ tm,
+                                         const int32_t *__restrict__ cand_t,
+                                         const int32_t *__restrict__ list, int64_t n_list,
+                                         const int32_t *__restrict__ k_lo,
+                                         const int32_t *__restrict__ k_hi,
+                                         const float *__restrict__ cols, int64_t cstride,
+                                         RTmpl &tp) {
+    const int64_t idx = (int64_t)blockIdx.x * RWG + threadIdx.x;
+    const int64_t slot = idx / G;
+    r.gl = (int)(idx % G);
+    r.has = slot < n_list;
+    r.ci = r.has ? list[slot] : 0;
+    r.kb = r.has ? k_lo[r.ci] : 1;
+    r.ke = r.has ? k_hi[r.ci] : 0;
+    tp = tm[r.has ? cand_t[r.ci] : 0];
+    const float *src = cols + tp.m_off;
+    const int first = r.kb + r.gl;
+    const int mine = r.ke >= first ? (r.ke - first) / G + 1 : 0;
+#pragma unroll
+    for (int j = 0; j < RB; j++) {
+        const int k = first + G * j;
+        if (j < mine) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) r.x[j][c] = src[c * cstride + k];
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; c++) r.x[j][c] = 0.0f;
+        }
+    }
+    r.nw = wave_max_int(mine);
+}
+
+template <int G>
+__device__ __forceinline__ Eval eval_regs(const RegSlot<G> &r, double t_data, double TA, double t0,
+                                          double tau) {
+    const double inv_tau = 1.0 / tau;
+    double s[2 * NC];
+#pragma unroll
+    for (int i = 0; i < 2 * NC; i++) s[i] = 0.0;
+    double u = fma((double)(r.kb + r.gl), TA, t_data - t0);  // t_k - t0
+    double w = exp(-u * inv_tau);
+    const double rg = exp(-(double)G * TA * inv_tau);
+    const double du = (double)G * TA;
+#pragma unroll
+    for (int j = 0; j < RB; j++) {
+        if (j < r.nw) {
+            const double w2 = w * w;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const double x = (c < 3 ? w2 : w) * (double)r.x[j][c];
+                s[c] += x;
+                s[NC + c] = fma(u, x, s[NC + c]);
+            }
+            w *= rg;
+            u += du;
+        }
+    }
+    group_sum<G>(s);
+    return eval_from_sums(s, inv_tau);
+}
+
+template <int G>
+__global__ __launch_bounds__(RWG) void refine_forward_regs_kernel(
+    const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
+    const int32_t *__restrict__ list, int64_t n_list, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, const double *__restrict__ tauv, double *__restrict__ twoF,
+    double *__restrict__ lnB, double *__restrict__ saved) {
+    RegSlot<G> r;
+    RTmpl tp;
+    reg_slot<G>(r, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
+    const double t0 = r.has ? t0v[r.ci] : 0.0;
+    const double tau = r.has ? tauv[r.ci] : 1.0;
+    const Eval e = eval_regs<G>(r, (double)tp.t_data, TA, t0, tau);
+    if (!r.has) return;
+    if (r.gl == 0) {
+        twoF[r.ci] = 2.0 * e.F;
+        if (lnB) lnB[r.ci] = LN70 + e.F;
+    }
+    if (saved)
+        for (int q = r.gl; q < NSAVED; q += G) saved[r.ci * NSAVED + q] = saved_value(e, q);
+}
+
+// The ascent of refine_ascent_kernel, step for step, on register supports.
+template <int G>
+__global__ __launch_bounds__(RWG) void refine_ascent_regs_kernel(
+    const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
+    const int32_t *__restrict__ list, int64_t n_list, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
+    const double *__restrict__ tau_hi, int n_steps, double step0, double *__restrict__ twoF,
+    int32_t *__restrict__ n_accept) {
+    RegSlot<G> r;
+    RTmpl tp;
+    reg_slot<G>(r, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
+    const double t_data = (double)tp.t_data;
+    const double t0 = r.has ? t0v[r.ci] : 0.0;
+    const double tau_in = r.has ? tauv[r.ci] : 1.0;
+    const bool live = r.has && tau_in > 0.0;
+    double tau = live ? tau_in : 1.0;
+    const double lo = (live && tau_lo) ? log(tau_lo[r.ci]) : -INFINITY;
+    const double hi = (live && tau_hi) ? log(tau_hi[r.ci]) : INFINITY;
+    if (!live) {  // an empty support: F = 2 and no move, as in the group kernel
+        r.kb = 1;
+        r.ke = 0;
+#pragma unroll
+        for (int j = 0; j < RB; j++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) r.x[j][c] = 0.0f;
+    }
+    Eval e = eval_regs<G>(r, t_data, TA, t0, tau);
+    double s = log(tau);
+    double g = tau * e.dFdtau;  // dF/dlog(tau)
+    double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
+    int acc = 0;
+    for (int step = 0; step < n_steps; step++) {
+        const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
+        const bool moved = sn != s;
+        const double taun = moved ? exp(sn) : tau;
+        const Eval en = eval_regs<G>(r, t_data, TA, t0, taun);
+        const bool accept = moved && en.F > e.F;
+        if (accept) {
+            s = sn;
+            tau = taun;
+            e = en;
+            g = tau * e.dFdtau;
+            acc++;
+        }
+        eta *= accept ? 1.5 : 0.5;
+    }
+    if (r.has && r.gl == 0) {
+        tauv[r.ci] = live ? tau : tau_in;
+        twoF[r.ci] = live ? 2.0 * e.F : NAN;
+        if (n_accept) n_accept[r.ci] = acc;
     }
 }
 
@@ -382,9 +560,14 @@ struct tcw_refine_plan {
     int64_t T = 0, n_cand = 0, total_bins = 0;
     uint32_t TAtom = 0;
     int lds_bytes = 0;
-    int64_t n_items = 0, n_bitems = 0;
-    DevBuf d_tmpl, d_items, d_bitems, d_klo, d_khi, d_ctmpl, d_coef;
+    int64_t n_items = 0, n_bitems = 0, n_long = 0, n_mid = 0;
+    // d_order: candidate indices; long supports (grouped by template, indexed
+    // by the items), then supports of SHORT1_MAX+1 .. SHORT4_MAX bins (four
+    // lanes each), then those of up to SHORT1_MAX bins (one lane each), the
+    // last two classes by decreasing length
+    DevBuf d_tmpl, d_items, d_bitems, d_klo, d_khi, d_ctmpl, d_coef, d_order;
     ~tcw_refine_plan() {
+        d_order.release();
         d_ctmpl.release();
         d_coef.release();
         d_tmpl.release();
@@ -396,6 +579,9 @@ struct tcw_refine_plan {
 };
 
 namespace {
+
+// blocks for n candidates at G lanes each
+unsigned reg_blocks(int64_t n, int G) { return (unsigned)((n * G + RWG - 1) / RWG); }
 
 int check_plan(const tcw_refine_plan *p) {
     if (!p) return fail(TCW_ERR_INVALID, "null refine plan");
@@ -418,6 +604,7 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
         return fail(TCW_ERR_INVALID, "cand_off must run from 0 to n_cand");
     std::vector<RTmpl> tm((size_t)d->T);
     std::vector<RItem> items;
+    std::vector<int32_t> order, mids, shorts;
     std::vector<BItem> bitems;
     int max_len = 0;
     for (int64_t t = 0; t < d->T; t++) {
@@ -434,13 +621,26 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
             if (d->k_lo[i] < 0 || d->k_hi[i] < d->k_lo[i] || d->k_hi[i] >= N)
                 return fail(TCW_ERR_INVALID, "candidate " + std::to_string(i) +
                                                  ": support outside its template's bins");
-        // items: consecutive candidates while the union fits in LDS
-        int64_t i = c0;
-        while (i < c1) {
-            int lo = d->k_lo[i], hi = d->k_hi[i];
+        // short supports go to the register lists; items take consecutive
+        // long ones (in caller order) while their union fits in LDS
+        const int64_t l0 = (int64_t)order.size();
+        for (int64_t i = c0; i < c1; i++) {
+            const int len = d->k_hi[i] - d->k_lo[i] + 1;
+            if (len <= SHORT1_MAX)
+                shorts.push_back((int32_t)i);
+            else if (len <= SHORT4_MAX)
+                mids.push_back((int32_t)i);
+            else
+                order.push_back((int32_t)i);
+        }
+        const int64_t l1 = (int64_t)order.size();
+        int64_t i = l0;
+        while (i < l1) {
+            int lo = d->k_lo[order[(size_t)i]], hi = d->k_hi[order[(size_t)i]];
             int64_t j = i + 1;
-            while (j < c1 && j - i < CAND_PER_ITEM) {
-                const int nlo = std::min(lo, d->k_lo[j]), nhi = std::max(hi, d->k_hi[j]);
+            while (j < l1 && j - i < CAND_PER_ITEM) {
+                const int32_t cj = order[(size_t)j];
+                const int nlo = std::min(lo, d->k_lo[cj]), nhi = std::max(hi, d->k_hi[cj]);
                 if (nhi - nlo + 1 > LDS_ATOMS_CAP && hi - lo + 1 <= LDS_ATOMS_CAP) break;
                 lo = nlo;
                 hi = nhi;
@@ -468,12 +668,23 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     p->lds_bytes = (int)(max_len * NC * sizeof(float));
     p->n_items = (int64_t)items.size();
     p->n_bitems = (int64_t)bitems.size();
+    p->n_long = (int64_t)order.size();
+    p->n_mid = (int64_t)mids.size();
+    // by decreasing length, so the lanes of a wave walk similar bin counts
+    const auto longer = [d](int32_t a, int32_t b) {
+        return d->k_hi[a] - d->k_lo[a] > d->k_hi[b] - d->k_lo[b];
+    };
+    std::stable_sort(mids.begin(), mids.end(), longer);
+    std::stable_sort(shorts.begin(), shorts.end(), longer);
+    order.insert(order.end(), mids.begin(), mids.end());
+    order.insert(order.end(), shorts.begin(), shorts.end());
     const size_t nc = (size_t)std::max<int64_t>(d->n_cand, 1);
     if ((rc = p->d_tmpl.ensure(tm.size() * sizeof(RTmpl))) ||
         (rc = p->d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(RItem))) ||
         (rc = p->d_bitems.ensure(bitems.size() * sizeof(BItem))) ||
         (rc = p->d_klo.ensure(nc * sizeof(int32_t))) || (rc = p->d_khi.ensure(nc * sizeof(int32_t))) ||
-        (rc = p->d_ctmpl.ensure(nc * sizeof(int32_t))) || (rc = p->d_coef.ensure(nc * sizeof(CandCoef)))) {
+        (rc = p->d_ctmpl.ensure(nc * sizeof(int32_t))) || (rc = p->d_coef.ensure(nc * sizeof(CandCoef))) ||
+        (rc = p->d_order.ensure(nc * sizeof(int32_t)))) {
         delete p;
         return rc;
     }
@@ -487,6 +698,8 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
         e = hipMemcpy(p->d_klo.p, d->k_lo, (size_t)d->n_cand * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && d->n_cand > 0)
         e = hipMemcpy(p->d_khi.p, d->k_hi, (size_t)d->n_cand * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && d->n_cand > 0)
+        e = hipMemcpy(p->d_order.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && d->n_cand > 0) {
         std::vector<int32_t> ct((size_t)d->n_cand);
         for (int64_t t = 0; t < d->T; t++)
@@ -515,10 +728,27 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
     if (p->n_cand == 0) return TCW_OK;
     if (!cols || !t0 || !tau || !twoF) return fail(TCW_ERR_INVALID, "tcw_refine_forward: null argument");
     if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
-    hipLaunchKernelGGL(refine_forward_kernel, dim3((unsigned)p->n_items), dim3(RWG), p->lds_bytes,
-                       (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p, (const RItem *)p->d_items.p,
-                       (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols, col_stride,
-                       (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
+    const int32_t *order = (const int32_t *)p->d_order.p;
+    if (p->n_items > 0)
+        hipLaunchKernelGGL(refine_forward_kernel<LONG_G>, dim3((unsigned)p->n_items),
+                           dim3(CAND_PER_ITEM * LONG_G), p->lds_bytes,
+                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           (const RItem *)p->d_items.p, order, (const int32_t *)p->d_klo.p,
+                           (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
+                           twoF, lnBtSG, saved);
+    const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
+    if (p->n_mid > 0)
+        hipLaunchKernelGGL(refine_forward_regs_kernel<4>, dim3(reg_blocks(p->n_mid, 4)), dim3(RWG), 0,
+                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           (const int32_t *)p->d_ctmpl.p, order + p->n_long, p->n_mid,
+                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
+                           col_stride, (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
+    if (n_short > 0)
+        hipLaunchKernelGGL(refine_forward_regs_kernel<1>, dim3(reg_blocks(n_short, 1)), dim3(RWG), 0,
+                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           (const int32_t *)p->d_ctmpl.p, order + p->n_long + p->n_mid, n_short,
+                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
+                           col_stride, (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
     TCW_HIP_TRY(hipGetLastError());
     return TCW_OK;
 }
@@ -557,10 +787,29 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
     if (n_steps < 0) return fail(TCW_ERR_INVALID, "n_steps must be >= 0");
     if (!(step0 > 0.0) || !std::isfinite(step0)) return fail(TCW_ERR_INVALID, "step0 must be > 0");
     if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
-    hipLaunchKernelGGL(refine_ascent_kernel, dim3((unsigned)p->n_items), dim3(RWG), p->lds_bytes,
-                       (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p, (const RItem *)p->d_items.p,
-                       (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols, col_stride,
-                       (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+    const int32_t *order = (const int32_t *)p->d_order.p;
+    if (p->n_items > 0)
+        hipLaunchKernelGGL(refine_ascent_kernel<LONG_G>, dim3((unsigned)p->n_items),
+                           dim3(CAND_PER_ITEM * LONG_G), p->lds_bytes,
+                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           (const RItem *)p->d_items.p, order, (const int32_t *)p->d_klo.p,
+                           (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
+                           tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+    const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
+    if (p->n_mid > 0)
+        hipLaunchKernelGGL(refine_ascent_regs_kernel<4>, dim3(reg_blocks(p->n_mid, 4)), dim3(RWG), 0,
+                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           (const int32_t *)p->d_ctmpl.p, order + p->n_long, p->n_mid,
+                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
+                           col_stride, (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0,
+                           twoF, n_accept);
+    if (n_short > 0)
+        hipLaunchKernelGGL(refine_ascent_regs_kernel<1>, dim3(reg_blocks(n_short, 1)), dim3(RWG), 0,
+                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           (const int32_t *)p->d_ctmpl.p, order + p->n_long + p->n_mid, n_short,
+                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
+                           col_stride, (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0,
+                           twoF, n_accept);
     TCW_HIP_TRY(hipGetLastError());
     return TCW_OK;
 }

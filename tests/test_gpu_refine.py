@@ -141,7 +141,10 @@ def test_forward_off_grid_vs_oracle(small):
     rng = np.random.default_rng(2)
     t0 = prob.t0_init.cpu().numpy() + rng.uniform(-0.4, 0.4, prob.n) * TA
     tau = prob.tau_init.cpu().numpy() * rng.uniform(0.8, 1.25, prob.n)
-    print(f"off-grid forward: {prob.n} candidates", flush=True)
+    L = prob.support_lengths
+    # supports of up to 32 bins run one candidate per lane, longer ones 16 lanes each
+    assert (L <= 32).any() and (L > 32).any()
+    print(f"off-grid forward: {prob.n} candidates, {(L <= 32).sum()} short", flush=True)
     _check_forward(prob, merged, work, t0, tau)
 
 
