@@ -1,6 +1,7 @@
 // tcw_refine.hip -- exponential-window candidates on a fixed atom support
 // (gfx950): 2F and lnBtSG with analytic first derivatives, the backward pass
-// to the merged atoms, and a multi-step gradient ascent in tau, in one launch.
+// to the merged atoms, and a multi-step gradient ascent in tau (all steps of a
+// candidate in one kernel).
 //
 // Window sums of a candidate (t0, tau) with support bins k_lo..k_hi:
 //   S_c = sum_k w_k^{p_c} x_{c,k},  w_k = exp(-(t_k - t0)/tau),  p = (2,2,2,1,1,1,1)
@@ -34,9 +35,9 @@ using tcwc::set_device;
 namespace {
 
 constexpr int NC = TCW_NCOL;
-constexpr int RWG = 256;             // threads per block of the register kernels
-constexpr int CAND_PER_ITEM = 16;    // candidates per LDS item (one block)
+constexpr int RWG = 256;             // threads per block
 constexpr int LONG_G = 16;           // lanes per candidate in the LDS kernels
+constexpr int CAND_PER_ITEM = RWG / LONG_G;  // candidates per LDS item (one block)
 constexpr int LDS_ATOMS_CAP = 2176;  // staged support bins: 7*4*2176 = 60928 B
 constexpr int BWD_BINS = 256;        // bins per backward block
 // Short supports are held in registers, RB bins per lane, loaded once per
@@ -258,7 +259,7 @@ __device__ __forceinline__ Slot lane_slot(const RItem &it, const int32_t *order,
 }
 
 template <int G>
-__global__ __launch_bounds__(CAND_PER_ITEM * G) void refine_forward_kernel(
+__global__ __launch_bounds__(RWG) void refine_forward_kernel(
     const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ order,
     const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
@@ -267,7 +268,9 @@ __global__ __launch_bounds__(CAND_PER_ITEM * G) void refine_forward_kernel(
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
     const RTmpl tp = tm[it.t];
-    const Src sr = stage_support<CAND_PER_ITEM * G>(it, tp, cols, cstride, lds);
+    const Src sr = stage_support<RWG>(it, tp, cols, cstride, lds);
+    // waves without a candidate leave after the staging barrier (no barrier follows)
+    if (((int)threadIdx.x & ~63) / G >= it.ncand) return;
     const int gl = (int)threadIdx.x % G;
     const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(CAND_PER_ITEM * G) void refine_forward_kernel(
 }
 
 template <int G>
-__global__ __launch_bounds__(CAND_PER_ITEM * G) void refine_ascent_kernel(
+__global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ order,
     const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
@@ -293,7 +296,9 @@ __global__ __launch_bounds__(CAND_PER_ITEM * G) void refine_ascent_kernel(
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
     const RTmpl tp = tm[it.t];
-    const Src sr = stage_support<CAND_PER_ITEM * G>(it, tp, cols, cstride, lds);
+    const Src sr = stage_support<RWG>(it, tp, cols, cstride, lds);
+    // waves without a candidate leave after the staging barrier (no barrier follows)
+    if (((int)threadIdx.x & ~63) / G >= it.ncand) return;
     const int gl = (int)threadIdx.x % G;
     const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
@@ -561,12 +566,20 @@ struct tcw_refine_plan {
     uint32_t TAtom = 0;
     int lds_bytes = 0;
     int64_t n_items = 0, n_bitems = 0, n_long = 0, n_mid = 0;
-    // d_order: candidate indices; long supports (grouped by template, indexed
-    // by the items), then supports of SHORT1_MAX+1 .. SHORT4_MAX bins (four
-    // lanes each), then those of up to SHORT1_MAX bins (one lane each), the
-    // last two classes by decreasing length
+    // d_order: candidate indices; long supports (grouped by template, in
+    // caller order, indexed by the items), then supports of
+    // SHORT1_MAX+1 .. SHORT4_MAX bins (four lanes each), then those of up to
+    // SHORT1_MAX bins (one lane each), the last two classes by decreasing
+    // length
     DevBuf d_tmpl, d_items, d_bitems, d_klo, d_khi, d_ctmpl, d_coef, d_order;
+    // the register kernels run on `side`, beside the LDS kernel on the
+    // caller's stream, so their blocks fill the LDS kernel's tail
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     ~tcw_refine_plan() {
+        if (side) (void)hipStreamDestroy(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         d_order.release();
         d_ctmpl.release();
         d_coef.release();
@@ -582,6 +595,25 @@ namespace {
 
 // blocks for n candidates at G lanes each
 unsigned reg_blocks(int64_t n, int G) { return (unsigned)((n * G + RWG - 1) / RWG); }
+
+// The stream for the register kernels: the plan's side stream, ordered after
+// the work already queued on st, when both kinds of kernel run; else st.
+int fork_side(tcw_refine_plan *p, hipStream_t st, hipStream_t *out) {
+    *out = st;
+    if (p->n_items == 0 || p->n_long == p->n_cand) return TCW_OK;
+    TCW_HIP_TRY(hipEventRecord(p->ev_fork, st));
+    TCW_HIP_TRY(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    *out = p->side;
+    return TCW_OK;
+}
+
+// Work queued on st after this waits for the side stream's kernels.
+int join_side(tcw_refine_plan *p, hipStream_t st, hipStream_t side) {
+    if (side == st) return TCW_OK;
+    TCW_HIP_TRY(hipEventRecord(p->ev_join, side));
+    TCW_HIP_TRY(hipStreamWaitEvent(st, p->ev_join, 0));
+    return TCW_OK;
+}
 
 int check_plan(const tcw_refine_plan *p) {
     if (!p) return fail(TCW_ERR_INVALID, "null refine plan");
@@ -622,7 +654,7 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
                 return fail(TCW_ERR_INVALID, "candidate " + std::to_string(i) +
                                                  ": support outside its template's bins");
         // short supports go to the register lists; items take consecutive
-        // long ones (in caller order) while their union fits in LDS
+        // long ones while their union fits in LDS
         const int64_t l0 = (int64_t)order.size();
         for (int64_t i = c0; i < c1; i++) {
             const int len = d->k_hi[i] - d->k_lo[i] + 1;
@@ -666,6 +698,10 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     p->TAtom = d->TAtom;
     p->total_bins = d->m_off[d->T];
     p->lds_bytes = (int)(max_len * NC * sizeof(float));
+    // widest unions first: their waves walk the longest serial chains, and
+    // blocks start roughly in index order
+    std::stable_sort(items.begin(), items.end(),
+                     [](const RItem &a, const RItem &b) { return a.u_len > b.u_len; });
     p->n_items = (int64_t)items.size();
     p->n_bitems = (int64_t)bitems.size();
     p->n_long = (int64_t)order.size();
@@ -688,7 +724,11 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
         delete p;
         return rc;
     }
-    hipError_t e = hipMemcpy(p->d_tmpl.p, tm.data(), tm.size() * sizeof(RTmpl), hipMemcpyHostToDevice);
+    hipError_t e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_tmpl.p, tm.data(), tm.size() * sizeof(RTmpl), hipMemcpyHostToDevice);
     if (e == hipSuccess && !items.empty())
         e = hipMemcpy(p->d_items.p, items.data(), items.size() * sizeof(RItem), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -729,28 +769,30 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
     if (!cols || !t0 || !tau || !twoF) return fail(TCW_ERR_INVALID, "tcw_refine_forward: null argument");
     if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
     const int32_t *order = (const int32_t *)p->d_order.p;
+    const hipStream_t st = (hipStream_t)stream;
+    hipStream_t side;
+    if ((rc = fork_side(p, st, &side))) return rc;
     if (p->n_items > 0)
         hipLaunchKernelGGL(refine_forward_kernel<LONG_G>, dim3((unsigned)p->n_items),
-                           dim3(CAND_PER_ITEM * LONG_G), p->lds_bytes,
-                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
                            (const RItem *)p->d_items.p, order, (const int32_t *)p->d_klo.p,
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
                            twoF, lnBtSG, saved);
     const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
     if (p->n_mid > 0)
         hipLaunchKernelGGL(refine_forward_regs_kernel<4>, dim3(reg_blocks(p->n_mid, 4)), dim3(RWG), 0,
-                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           side, (const RTmpl *)p->d_tmpl.p,
                            (const int32_t *)p->d_ctmpl.p, order + p->n_long, p->n_mid,
                            (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
                            col_stride, (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
     if (n_short > 0)
         hipLaunchKernelGGL(refine_forward_regs_kernel<1>, dim3(reg_blocks(n_short, 1)), dim3(RWG), 0,
-                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           side, (const RTmpl *)p->d_tmpl.p,
                            (const int32_t *)p->d_ctmpl.p, order + p->n_long + p->n_mid, n_short,
                            (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
                            col_stride, (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
     TCW_HIP_TRY(hipGetLastError());
-    return TCW_OK;
+    return join_side(p, st, side);
 }
 
 int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau, const double *saved,
@@ -788,30 +830,32 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
     if (!(step0 > 0.0) || !std::isfinite(step0)) return fail(TCW_ERR_INVALID, "step0 must be > 0");
     if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
     const int32_t *order = (const int32_t *)p->d_order.p;
+    const hipStream_t st = (hipStream_t)stream;
+    hipStream_t side;
+    if ((rc = fork_side(p, st, &side))) return rc;
     if (p->n_items > 0)
         hipLaunchKernelGGL(refine_ascent_kernel<LONG_G>, dim3((unsigned)p->n_items),
-                           dim3(CAND_PER_ITEM * LONG_G), p->lds_bytes,
-                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
                            (const RItem *)p->d_items.p, order, (const int32_t *)p->d_klo.p,
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
                            tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
     const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
     if (p->n_mid > 0)
         hipLaunchKernelGGL(refine_ascent_regs_kernel<4>, dim3(reg_blocks(p->n_mid, 4)), dim3(RWG), 0,
-                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           side, (const RTmpl *)p->d_tmpl.p,
                            (const int32_t *)p->d_ctmpl.p, order + p->n_long, p->n_mid,
                            (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
                            col_stride, (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0,
                            twoF, n_accept);
     if (n_short > 0)
         hipLaunchKernelGGL(refine_ascent_regs_kernel<1>, dim3(reg_blocks(n_short, 1)), dim3(RWG), 0,
-                           (hipStream_t)stream, (const RTmpl *)p->d_tmpl.p,
+                           side, (const RTmpl *)p->d_tmpl.p,
                            (const int32_t *)p->d_ctmpl.p, order + p->n_long + p->n_mid, n_short,
                            (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
                            col_stride, (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0,
                            twoF, n_accept);
     TCW_HIP_TRY(hipGetLastError());
-    return TCW_OK;
+    return join_side(p, st, side);
 }
 
 }  // extern "C"
