@@ -38,7 +38,7 @@ constexpr int NC = TCW_NCOL;
 constexpr int RWG = 256;             // threads per block
 constexpr int LONG_G = 16;           // lanes per candidate in the LDS kernels
 constexpr int CAND_PER_ITEM = RWG / LONG_G;  // candidates per LDS item (one block)
-constexpr int LDS_ATOMS_CAP = 2176;  // staged support bins: 7*4*2176 = 60928 B
+constexpr int LDS_ATOMS_CAP = 1904;  // staged support bins: 32 B each, 60928 B
 constexpr int BWD_BINS = 256;        // bins per backward block
 // Short supports are held in registers, RB bins per lane, loaded once per
 // launch: up to RB bins on one lane per candidate, up to 4*RB bins on four
@@ -53,6 +53,7 @@ constexpr double COND_MAX = 1e4;
 constexpr double QCOND = (COND_MAX + 1.0) * (COND_MAX + 1.0) / COND_MAX;  // cond < K <=> T^2 < Q D
 constexpr double LN70 = 4.248495242049359;  // log(70)
 constexpr int NSAVED = TCW_REFINE_SAVED;
+static_assert(NSAVED == NC + 3, "saved row: dF/dt0, dF/dtau, dF/dS_c, F");
 
 struct RTmpl {
     int64_t m_off;             // first merged bin
@@ -122,13 +123,39 @@ __device__ __forceinline__ void group_sum(double (&s)[N]) {
     }
 }
 
+// Bin k's seven atoms from an item's LDS copy: bin-major, 8 floats a bin (two
+// 128-bit reads), bins from `base` on.
+struct LdsBins {
+    const float4 *b;
+    int base;
+    __device__ __forceinline__ void get(int k, float (&x)[NC]) const {
+        const float4 p = b[2 * (k - base)], q = b[2 * (k - base) + 1];
+        x[0] = p.x;
+        x[1] = p.y;
+        x[2] = p.z;
+        x[3] = p.w;
+        x[4] = q.x;
+        x[5] = q.y;
+        x[6] = q.z;
+    }
+};
+
+// Bin k's seven atoms from the template's columns in global memory.
+struct ColBins {
+    const float *g;
+    int64_t stride;
+    __device__ __forceinline__ void get(int k, float (&x)[NC]) const {
+#pragma unroll
+        for (int c = 0; c < NC; c++) x[c] = g[c * stride + k];
+    }
+};
+
 // Window sums of one candidate per G-lane group: s[0..6] = S_c, s[7..13] =
-// U_c.  Atom (c, k) of the template is src[c*sstride + k - sbase] (src in
-// LDS or in global memory); an empty support (kb > ke) gives zeros.
-template <int G, typename P>
-__device__ __forceinline__ void group_sums(P src, int64_t sstride, int sbase, int kb, int ke,
-                                           double t_data, double TA, double t0, double inv_tau,
-                                           int gl, double (&s)[2 * NC]) {
+// U_c; an empty support (kb > ke) gives zeros.
+template <int G, typename Bins>
+__device__ __forceinline__ void group_sums(const Bins &src, int kb, int ke, double t_data,
+                                           double TA, double t0, double inv_tau, int gl,
+                                           double (&s)[2 * NC]) {
 #pragma unroll
     for (int i = 0; i < 2 * NC; i++) s[i] = 0.0;
     const int k0 = kb + gl;
@@ -136,12 +163,15 @@ __device__ __forceinline__ void group_sums(P src, int64_t sstride, int sbase, in
     double w = exp(-u * inv_tau);
     const double rg = exp(-(double)G * TA * inv_tau);
     const double du = (double)G * TA;
+    // two bins a trip: their LDS reads overlap
+#pragma unroll 2
     for (int k = k0; k <= ke; k += G) {
-        const int j = k - sbase;
+        float a[NC];
+        src.get(k, a);
         const double w2 = w * w;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const double x = (c < 3 ? w2 : w) * (double)src[c * sstride + j];
+            const double x = (c < 3 ? w2 : w) * (double)a[c];
             s[c] += x;
             s[NC + c] = fma(u, x, s[NC + c]);
         }
@@ -184,15 +214,14 @@ __device__ __forceinline__ Eval eval_from_sums(const double (&s)[2 * NC], double
     return e;
 }
 
-// Entry q of a candidate's saved row: dF/dt0, dF/dtau, dF/dS_c (7), F.
-__device__ __forceinline__ double saved_value(const Eval &e, int q) {
-    double v = e.F;  // q == 9
-    if (q == 0) v = e.dFdt0;
-    if (q == 1) v = e.dFdtau;
+// A candidate's saved row: dF/dt0, dF/dtau, dF/dS_c (7), F (fixed indices,
+// so the values stay in registers).
+__device__ __forceinline__ void write_saved(double *__restrict__ row, const Eval &e) {
+    row[0] = e.dFdt0;
+    row[1] = e.dFdtau;
 #pragma unroll
-    for (int c = 0; c < NC; c++)
-        if (q == 2 + c) v = e.g[c];
-    return v;
+    for (int c = 0; c < NC; c++) row[2 + c] = e.g[c];
+    row[2 + NC] = e.F;
 }
 
 // The atoms of an item: its LDS copy of the support union, or the template's
@@ -200,7 +229,7 @@ __device__ __forceinline__ double saved_value(const Eval &e, int q) {
 struct Src {
     const float *g;  // global: template base
     int64_t gstride;
-    int lstride, lbase;
+    int lbase;
     bool in_lds;
 };
 
@@ -211,14 +240,14 @@ __device__ __forceinline__ Eval eval_group(const Src &sr, const float *lds, int 
     const double inv_tau = 1.0 / tau;
     double s[2 * NC];
     if (sr.in_lds)  // block-uniform
-        group_sums<G>(lds, sr.lstride, sr.lbase, kb, ke, t_data, TA, t0, inv_tau, gl, s);
+        group_sums<G>(LdsBins{(const float4 *)lds, sr.lbase}, kb, ke, t_data, TA, t0, inv_tau, gl, s);
     else
-        group_sums<G>(sr.g, sr.gstride, 0, kb, ke, t_data, TA, t0, inv_tau, gl, s);
+        group_sums<G>(ColBins{sr.g, sr.gstride}, kb, ke, t_data, TA, t0, inv_tau, gl, s);
     return eval_from_sums(s, inv_tau);
 }
 
-// Stage the item's support union (component-major) into LDS when it fits
-// (NT threads per block).
+// Stage the item's support union into LDS when it fits (bin-major, 8 floats a
+// bin; NT threads per block).
 template <int NT>
 __device__ __forceinline__ Src stage_support(const RItem &it, const RTmpl &tp,
                                              const float *__restrict__ cols, int64_t cstride,
@@ -226,13 +255,18 @@ __device__ __forceinline__ Src stage_support(const RItem &it, const RTmpl &tp,
     Src sr;
     sr.g = cols + tp.m_off;
     sr.gstride = cstride;
-    sr.lstride = it.u_len;
     sr.lbase = it.u_lo;
     sr.in_lds = it.in_lds != 0;
     if (sr.in_lds) {
         const float *g = cols + tp.m_off + it.u_lo;
-        for (int c = 0; c < NC; c++)
-            for (int k = threadIdx.x; k < it.u_len; k += NT) lds[c * it.u_len + k] = g[c * cstride + k];
+        float4 *b = (float4 *)lds;
+        for (int k = threadIdx.x; k < it.u_len; k += NT) {
+            float x[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) x[c] = g[c * cstride + k];
+            b[2 * k] = make_float4(x[0], x[1], x[2], x[3]);
+            b[2 * k + 1] = make_float4(x[4], x[5], x[6], 0.0f);
+        }
         __syncthreads();
     }
     return sr;
@@ -280,9 +314,8 @@ __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     if (gl == 0) {
         twoF[sl.ci] = 2.0 * e.F;
         if (lnB) lnB[sl.ci] = LN70 + e.F;
+        if (saved) write_saved(saved + sl.ci * NSAVED, e);
     }
-    if (saved)
-        for (int q = gl; q < NSAVED; q += G) saved[sl.ci * NSAVED + q] = saved_value(e, q);
 }
 
 template <int G>
@@ -435,9 +468,8 @@ __global__ __launch_bounds__(RWG) void refine_forward_regs_kernel(
     if (r.gl == 0) {
         twoF[r.ci] = 2.0 * e.F;
         if (lnB) lnB[r.ci] = LN70 + e.F;
+        if (saved) write_saved(saved + r.ci * NSAVED, e);
     }
-    if (saved)
-        for (int q = r.gl; q < NSAVED; q += G) saved[r.ci * NSAVED + q] = saved_value(e, q);
 }
 
 // The ascent of refine_ascent_kernel, step for step, on register supports.
@@ -697,7 +729,7 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     p->n_cand = d->n_cand;
     p->TAtom = d->TAtom;
     p->total_bins = d->m_off[d->T];
-    p->lds_bytes = (int)(max_len * NC * sizeof(float));
+    p->lds_bytes = (int)(max_len * 2 * sizeof(float4));
     // widest unions first: their waves walk the longest serial chains, and
     // blocks start roughly in index order
     std::stable_sort(items.begin(), items.end(),
