@@ -379,7 +379,7 @@ def run_refine(args):
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "refine_ascent_kernel<16> + refine_ascent_regs_kernel<4>, <1> (two streams)",
+            "kernel": "refine_ascent_kernel<16> + refine_ascent_regs_kernel (two streams)",
             "achieved": achieved,
             "peak": FP64_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",

@@ -390,14 +390,14 @@ __device__ __forceinline__ int wave_max_int(int v) {
 
 // Called by every lane of the block (the wave maximum needs them all).
 template <int G>
-__device__ __forceinline__ void reg_slot(RegSlot<G> &r, const RTmpl *__restrict__ tm,
+__device__ __forceinline__ void reg_slot(RegSlot<G> &r, int64_t blk, const RTmpl *__restrict__ tm,
                                          const int32_t *__restrict__ cand_t,
                                          const int32_t *__restrict__ list, int64_t n_list,
                                          const int32_t *__restrict__ k_lo,
                                          const int32_t *__restrict__ k_hi,
                                          const float *__restrict__ cols, int64_t cstride,
                                          RTmpl &tp) {
-    const int64_t idx = (int64_t)blockIdx.x * RWG + threadIdx.x;
+    const int64_t idx = blk * RWG + threadIdx.x;
     const int64_t slot = idx / G;
     r.gl = (int)(idx % G);
     r.has = slot < n_list;
@@ -451,16 +451,17 @@ __device__ __forceinline__ Eval eval_regs(const RegSlot<G> &r, double t_data, do
     return eval_from_sums(s, inv_tau);
 }
 
+// Block blk of a register class (G lanes per candidate of `list`).
 template <int G>
-__global__ __launch_bounds__(RWG) void refine_forward_regs_kernel(
-    const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
+__device__ __forceinline__ void forward_regs(
+    int64_t blk, const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
     const int32_t *__restrict__ list, int64_t n_list, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
     const double *__restrict__ t0v, const double *__restrict__ tauv, double *__restrict__ twoF,
     double *__restrict__ lnB, double *__restrict__ saved) {
     RegSlot<G> r;
     RTmpl tp;
-    reg_slot<G>(r, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
+    reg_slot<G>(r, blk, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
     const double t0 = r.has ? t0v[r.ci] : 0.0;
     const double tau = r.has ? tauv[r.ci] : 1.0;
     const Eval e = eval_regs<G>(r, (double)tp.t_data, TA, t0, tau);
@@ -474,8 +475,8 @@ __global__ __launch_bounds__(RWG) void refine_forward_regs_kernel(
 
 // The ascent of refine_ascent_kernel, step for step, on register supports.
 template <int G>
-__global__ __launch_bounds__(RWG) void refine_ascent_regs_kernel(
-    const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
+__device__ __forceinline__ void ascent_regs(
+    int64_t blk, const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
     const int32_t *__restrict__ list, int64_t n_list, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
     const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_regs_kernel(
     int32_t *__restrict__ n_accept) {
     RegSlot<G> r;
     RTmpl tp;
-    reg_slot<G>(r, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
+    reg_slot<G>(r, blk, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
     const double t_data = (double)tp.t_data;
     const double t0 = r.has ? t0v[r.ci] : 0.0;
     const double tau_in = r.has ? tauv[r.ci] : 1.0;
@@ -524,6 +525,40 @@ __global__ __launch_bounds__(RWG) void refine_ascent_regs_kernel(
         twoF[r.ci] = live ? 2.0 * e.F : NAN;
         if (n_accept) n_accept[r.ci] = acc;
     }
+}
+
+// One launch for both register classes: blocks below mid_blocks take the
+// four-lane list (supports of SHORT1_MAX+1 .. SHORT4_MAX bins), the rest the
+// one-lane list.
+__global__ __launch_bounds__(RWG) void refine_forward_regs_kernel(
+    const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
+    const int32_t *__restrict__ mids, int64_t n_mid, unsigned mid_blocks,
+    const int32_t *__restrict__ shorts, int64_t n_short, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, const double *__restrict__ tauv, double *__restrict__ twoF,
+    double *__restrict__ lnB, double *__restrict__ saved) {
+    if (blockIdx.x < mid_blocks)  // block-uniform
+        forward_regs<4>(blockIdx.x, tm, cand_t, mids, n_mid, k_lo, k_hi, cols, cstride, TA, t0v,
+                        tauv, twoF, lnB, saved);
+    else
+        forward_regs<1>(blockIdx.x - mid_blocks, tm, cand_t, shorts, n_short, k_lo, k_hi, cols,
+                        cstride, TA, t0v, tauv, twoF, lnB, saved);
+}
+
+__global__ __launch_bounds__(RWG) void refine_ascent_regs_kernel(
+    const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
+    const int32_t *__restrict__ mids, int64_t n_mid, unsigned mid_blocks,
+    const int32_t *__restrict__ shorts, int64_t n_short, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
+    const double *__restrict__ tau_hi, int n_steps, double step0, double *__restrict__ twoF,
+    int32_t *__restrict__ n_accept) {
+    if (blockIdx.x < mid_blocks)  // block-uniform
+        ascent_regs<4>(blockIdx.x, tm, cand_t, mids, n_mid, k_lo, k_hi, cols, cstride, TA, t0v,
+                       tauv, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+    else
+        ascent_regs<1>(blockIdx.x - mid_blocks, tm, cand_t, shorts, n_short, k_lo, k_hi, cols,
+                       cstride, TA, t0v, tauv, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
 }
 
 // Per candidate, once per backward: g = 2 g_twoF + g_lnBtSG, the t0 / tau
@@ -811,17 +846,12 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
                            twoF, lnBtSG, saved);
     const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
-    if (p->n_mid > 0)
-        hipLaunchKernelGGL(refine_forward_regs_kernel<4>, dim3(reg_blocks(p->n_mid, 4)), dim3(RWG), 0,
-                           side, (const RTmpl *)p->d_tmpl.p,
-                           (const int32_t *)p->d_ctmpl.p, order + p->n_long, p->n_mid,
-                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
-                           col_stride, (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
-    if (n_short > 0)
-        hipLaunchKernelGGL(refine_forward_regs_kernel<1>, dim3(reg_blocks(n_short, 1)), dim3(RWG), 0,
-                           side, (const RTmpl *)p->d_tmpl.p,
-                           (const int32_t *)p->d_ctmpl.p, order + p->n_long + p->n_mid, n_short,
-                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
+    const unsigned mid_blocks = reg_blocks(p->n_mid, 4), short_blocks = reg_blocks(n_short, 1);
+    if (mid_blocks + short_blocks > 0)
+        hipLaunchKernelGGL(refine_forward_regs_kernel, dim3(mid_blocks + short_blocks), dim3(RWG), 0,
+                           side, (const RTmpl *)p->d_tmpl.p, (const int32_t *)p->d_ctmpl.p,
+                           order + p->n_long, p->n_mid, mid_blocks, order + p->n_long + p->n_mid,
+                           n_short, (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
                            col_stride, (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
     TCW_HIP_TRY(hipGetLastError());
     return join_side(p, st, side);
@@ -872,18 +902,12 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
                            tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
     const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
-    if (p->n_mid > 0)
-        hipLaunchKernelGGL(refine_ascent_regs_kernel<4>, dim3(reg_blocks(p->n_mid, 4)), dim3(RWG), 0,
-                           side, (const RTmpl *)p->d_tmpl.p,
-                           (const int32_t *)p->d_ctmpl.p, order + p->n_long, p->n_mid,
-                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
-                           col_stride, (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0,
-                           twoF, n_accept);
-    if (n_short > 0)
-        hipLaunchKernelGGL(refine_ascent_regs_kernel<1>, dim3(reg_blocks(n_short, 1)), dim3(RWG), 0,
-                           side, (const RTmpl *)p->d_tmpl.p,
-                           (const int32_t *)p->d_ctmpl.p, order + p->n_long + p->n_mid, n_short,
-                           (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
+    const unsigned mid_blocks = reg_blocks(p->n_mid, 4), short_blocks = reg_blocks(n_short, 1);
+    if (mid_blocks + short_blocks > 0)
+        hipLaunchKernelGGL(refine_ascent_regs_kernel, dim3(mid_blocks + short_blocks), dim3(RWG), 0,
+                           side, (const RTmpl *)p->d_tmpl.p, (const int32_t *)p->d_ctmpl.p,
+                           order + p->n_long, p->n_mid, mid_blocks, order + p->n_long + p->n_mid,
+                           n_short, (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
                            col_stride, (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0,
                            twoF, n_accept);
     TCW_HIP_TRY(hipGetLastError());
