@@ -277,17 +277,19 @@ int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau,
                         double *grad_tau, void *stream);
 
 /* n_steps of gradient ascent of F in log(tau) per candidate, on the fixed
- * support, in one pass per candidate: a step proposes s' = s + eta dF/ds (s = log tau,
- * clamped to [log tau_lo, log tau_hi]; tau_lo/tau_hi may be NULL for no
- * bound), is accepted if F rises (eta *= 1.5) and rejected otherwise
+ * support (all steps of a candidate in one kernel): a step proposes
+ * s' = s + eta dF/ds (s = log tau, clamped to [log tau_lo, log tau_hi];
+ * tau_lo/tau_hi may be NULL for no bound), is accepted if F rises
+ * (eta *= 1.5) and rejected otherwise
  * (eta *= 0.5); the first step moves s by step0.  t0 is not moved (dF/dt0 = 0
- * on a fixed support).  tau is updated in place; twoF receives the final 2F
+ * on a fixed support).  Starts from tau_in and writes the final tau to
+ * tau_out (the two may be the same array); twoF receives the final 2F
  * and n_accept (may be NULL) the accepted steps.  Asynchronous on `stream`,
  * as tcw_refine_forward (same side stream). */
 int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
-                      const double *t0, double *tau, const double *tau_lo,
-                      const double *tau_hi, int n_steps, double step0, double *twoF,
-                      int32_t *n_accept, void *stream);
+                      const double *t0, const double *tau_in, double *tau_out,
+                      const double *tau_lo, const double *tau_hi, int n_steps, double step0,
+                      double *twoF, int32_t *n_accept, void *stream);
 
 #ifdef __cplusplus
 }

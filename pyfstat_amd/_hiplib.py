@@ -199,8 +199,8 @@ def load():
     lib.tcw_refine_backward.restype = ctypes.c_int
     lib.tcw_refine_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp]
     lib.tcw_refine_ascent.restype = ctypes.c_int
-    lib.tcw_refine_ascent.argtypes = [vp, vp, i64, vp, vp, vp, vp, ctypes.c_int, ctypes.c_double,
-                                      vp, vp, vp]
+    lib.tcw_refine_ascent.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, ctypes.c_int,
+                                      ctypes.c_double, vp, vp, vp]
     _lib = lib
     return lib
 

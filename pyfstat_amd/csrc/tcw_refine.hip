@@ -323,9 +323,9 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ order,
     const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
-    const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
-    const double *__restrict__ tau_hi, int n_steps, double step0, double *__restrict__ twoF,
-    int32_t *__restrict__ n_accept) {
+    const double *__restrict__ t0v, const double *tau_iv, double *tau_ov,
+    const double *__restrict__ tau_lo, const double *__restrict__ tau_hi, int n_steps,
+    double step0, double *__restrict__ twoF, int32_t *__restrict__ n_accept) {
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
     const RTmpl tp = tm[it.t];
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     const int gl = (int)threadIdx.x % G;
     const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
-    const double tau_in = sl.has ? tauv[sl.ci] : 1.0;
+    const double tau_in = sl.has ? tau_iv[sl.ci] : 1.0;
     const bool live = sl.has && tau_in > 0.0;
     double tau = live ? tau_in : 1.0;
     const double lo = (live && tau_lo) ? log(tau_lo[sl.ci]) : -INFINITY;
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
         eta *= accept ? 1.5 : 0.5;
     }
     if (sl.has && gl == 0) {
-        tauv[sl.ci] = live ? tau : tau_in;
+        tau_ov[sl.ci] = live ? tau : tau_in;
         twoF[sl.ci] = live ? 2.0 * e.F : NAN;
         if (n_accept) n_accept[sl.ci] = acc;
     }
@@ -479,15 +479,15 @@ __device__ __forceinline__ void ascent_regs(
     int64_t blk, const RTmpl *__restrict__ tm, const int32_t *__restrict__ cand_t,
     const int32_t *__restrict__ list, int64_t n_list, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
-    const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
-    const double *__restrict__ tau_hi, int n_steps, double step0, double *__restrict__ twoF,
-    int32_t *__restrict__ n_accept) {
+    const double *__restrict__ t0v, const double *tau_iv, double *tau_ov,
+    const double *__restrict__ tau_lo, const double *__restrict__ tau_hi, int n_steps,
+    double step0, double *__restrict__ twoF, int32_t *__restrict__ n_accept) {
     RegSlot<G> r;
     RTmpl tp;
     reg_slot<G>(r, blk, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
     const double t_data = (double)tp.t_data;
     const double t0 = r.has ? t0v[r.ci] : 0.0;
-    const double tau_in = r.has ? tauv[r.ci] : 1.0;
+    const double tau_in = r.has ? tau_iv[r.ci] : 1.0;
     const bool live = r.has && tau_in > 0.0;
     double tau = live ? tau_in : 1.0;
     const double lo = (live && tau_lo) ? log(tau_lo[r.ci]) : -INFINITY;
@@ -521,7 +521,7 @@ __device__ __forceinline__ void ascent_regs(
         eta *= accept ? 1.5 : 0.5;
     }
     if (r.has && r.gl == 0) {
-        tauv[r.ci] = live ? tau : tau_in;
+        tau_ov[r.ci] = live ? tau : tau_in;
         twoF[r.ci] = live ? 2.0 * e.F : NAN;
         if (n_accept) n_accept[r.ci] = acc;
     }
@@ -550,15 +550,16 @@ __global__ __launch_bounds__(RWG) void refine_ascent_regs_kernel(
     const int32_t *__restrict__ mids, int64_t n_mid, unsigned mid_blocks,
     const int32_t *__restrict__ shorts, int64_t n_short, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
-    const double *__restrict__ t0v, double *__restrict__ tauv, const double *__restrict__ tau_lo,
-    const double *__restrict__ tau_hi, int n_steps, double step0, double *__restrict__ twoF,
-    int32_t *__restrict__ n_accept) {
+    const double *__restrict__ t0v, const double *tau_iv, double *tau_ov,
+    const double *__restrict__ tau_lo, const double *__restrict__ tau_hi, int n_steps,
+    double step0, double *__restrict__ twoF, int32_t *__restrict__ n_accept) {
     if (blockIdx.x < mid_blocks)  // block-uniform
         ascent_regs<4>(blockIdx.x, tm, cand_t, mids, n_mid, k_lo, k_hi, cols, cstride, TA, t0v,
-                       tauv, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+                       tau_iv, tau_ov, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
     else
         ascent_regs<1>(blockIdx.x - mid_blocks, tm, cand_t, shorts, n_short, k_lo, k_hi, cols,
-                       cstride, TA, t0v, tauv, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+                       cstride, TA, t0v, tau_iv, tau_ov, tau_lo, tau_hi, n_steps, step0, twoF,
+                       n_accept);
 }
 
 // Per candidate, once per backward: g = 2 g_twoF + g_lnBtSG, the t0 / tau
@@ -882,12 +883,14 @@ int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau,
 }
 
 int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride, const double *t0,
-                      double *tau, const double *tau_lo, const double *tau_hi, int n_steps,
-                      double step0, double *twoF, int32_t *n_accept, void *stream) {
+                      const double *tau_in, double *tau_out, const double *tau_lo,
+                      const double *tau_hi, int n_steps, double step0, double *twoF,
+                      int32_t *n_accept, void *stream) {
     int rc = check_plan(p);
     if (rc) return rc;
     if (p->n_cand == 0) return TCW_OK;
-    if (!cols || !t0 || !tau || !twoF) return fail(TCW_ERR_INVALID, "tcw_refine_ascent: null argument");
+    if (!cols || !t0 || !tau_in || !tau_out || !twoF)
+        return fail(TCW_ERR_INVALID, "tcw_refine_ascent: null argument");
     if (n_steps < 0) return fail(TCW_ERR_INVALID, "n_steps must be >= 0");
     if (!(step0 > 0.0) || !std::isfinite(step0)) return fail(TCW_ERR_INVALID, "step0 must be > 0");
     if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
@@ -899,8 +902,8 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
         hipLaunchKernelGGL(refine_ascent_kernel<LONG_G>, dim3((unsigned)p->n_items),
                            dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
                            (const RItem *)p->d_items.p, order, (const int32_t *)p->d_klo.p,
-                           (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
-                           tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+                           (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0,
+                           tau_in, tau_out, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
     const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
     const unsigned mid_blocks = reg_blocks(p->n_mid, 4), short_blocks = reg_blocks(n_short, 1);
     if (mid_blocks + short_blocks > 0)
@@ -908,8 +911,8 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
                            side, (const RTmpl *)p->d_tmpl.p, (const int32_t *)p->d_ctmpl.p,
                            order + p->n_long, p->n_mid, mid_blocks, order + p->n_long + p->n_mid,
                            n_short, (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
-                           col_stride, (double)p->TAtom, t0, tau, tau_lo, tau_hi, n_steps, step0,
-                           twoF, n_accept);
+                           col_stride, (double)p->TAtom, t0, tau_in, tau_out, tau_lo, tau_hi,
+                           n_steps, step0, twoF, n_accept);
     TCW_HIP_TRY(hipGetLastError());
     return join_side(p, st, side);
 }

@@ -314,18 +314,20 @@ class ExpRefineProblem:
         return ExpRefine.apply(cols, t0, tau, self)
 
     def ascent(self, cols, tau=None, tau_lo=None, tau_hi=None, n_steps=20, step0=0.02):
-        """n_steps of gradient ascent of F in log(tau) per candidate, in one
-        launch.  Returns (tau, twoF, n_accept); t0 stays at ``t0_init``."""
+        """n_steps of gradient ascent of F in log(tau) per candidate (all steps
+        of a candidate in one kernel).  Returns (tau, twoF, n_accept) as new
+        tensors; t0 stays at ``t0_init`` and the tau passed in is not changed."""
         torch = _torch()
-        tau = (self.tau_init if tau is None else tau).clone()
-        self._check_inputs(cols, self.t0_init, tau)
+        tau_in = self.tau_init if tau is None else tau
+        self._check_inputs(cols, self.t0_init, tau_in)
+        tau = torch.empty_like(tau_in)
         twoF = torch.empty(self.n, dtype=torch.float64, device=self.device)
         nacc = torch.empty(self.n, dtype=torch.int32, device=self.device)
         lo = tau_lo.to(torch.float64).contiguous() if tau_lo is not None else None
         hi = tau_hi.to(torch.float64).contiguous() if tau_hi is not None else None
         _hiplib.check(self.lib.tcw_refine_ascent(
-            self.plan, _ptr(cols), cols.shape[1], _ptr(self.t0_init), _ptr(tau), _ptr(lo),
-            _ptr(hi), int(n_steps), float(step0), _ptr(twoF), _ptr(nacc),
+            self.plan, _ptr(cols), cols.shape[1], _ptr(self.t0_init), _ptr(tau_in), _ptr(tau),
+            _ptr(lo), _ptr(hi), int(n_steps), float(step0), _ptr(twoF), _ptr(nacc),
             _stream(torch, self.device)), "tcw_refine_ascent")
         return tau, twoF, nacc
 

@@ -142,8 +142,9 @@ def test_forward_off_grid_vs_oracle(small):
     t0 = prob.t0_init.cpu().numpy() + rng.uniform(-0.4, 0.4, prob.n) * TA
     tau = prob.tau_init.cpu().numpy() * rng.uniform(0.8, 1.25, prob.n)
     L = prob.support_lengths
-    # supports of up to 32 bins run one candidate per lane, longer ones 16 lanes each
-    assert (L <= 32).any() and (L > 32).any()
+    # supports of up to 8 bins run one candidate per lane, up to 32 bins four
+    # lanes each (both from registers), longer ones 16 lanes each from LDS
+    assert (L <= 8).any() and ((L > 8) & (L <= 32)).any() and (L > 32).any()
     print(f"off-grid forward: {prob.n} candidates, {(L <= 32).sum()} short", flush=True)
     _check_forward(prob, merged, work, t0, tau)
 
@@ -231,7 +232,17 @@ def test_ascent_monotone_bounded_and_converged(small):
     lo = torch.clamp(tau0 - TA, min=float(TA))
     hi = tau0 + TA
     F_init = prob.forward_raw(merged.cols, prob.t0_init, tau0, want_saved=False)[0]
+    tau0_before = tau0.clone()
     tau, twoF, nacc = prob.ascent(merged.cols, tau_lo=lo, tau_hi=hi, n_steps=20)
+    assert torch.equal(tau0, tau0_before)  # the start point is not overwritten
+    # in place through the C ABI (tau_in == tau_out): the same bits
+    tau_ip = tau0.clone()
+    twoF_ip = torch.empty_like(twoF)
+    assert prob.lib.tcw_refine_ascent(
+        prob.plan, merged.cols.data_ptr(), merged.cols.shape[1], prob.t0_init.data_ptr(),
+        tau_ip.data_ptr(), tau_ip.data_ptr(), lo.data_ptr(), hi.data_ptr(), 20, 0.02,
+        twoF_ip.data_ptr(), None, torch.cuda.current_stream().cuda_stream) == 0
+    assert torch.equal(tau_ip, tau) and torch.equal(twoF_ip, twoF)
     tau_h, twoF_h = tau.cpu().numpy(), twoF.cpu().numpy()
     # never below the start (the forward and ascent kernels share the evaluation;
     # allow last-bit differences of separately compiled copies)
