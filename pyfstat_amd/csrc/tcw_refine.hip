@@ -150,8 +150,8 @@ struct ColBins {
     }
 };
 
-// Window sums of one candidate per G-lane group: s[0..6] = S_c, s[7..13] =
-// U_c; an empty support (kb > ke) gives zeros.
+// A lane's share of the window sums of its group's candidate: s[0..6] = S_c,
+// s[7..13] = U_c over the lane's bins; an empty support (kb > ke) gives zeros.
 template <int G, typename Bins>
 __device__ __forceinline__ void group_sums(const Bins &src, int kb, int ke, double t_data,
                                            double TA, double t0, double inv_tau, int gl,
@@ -178,14 +178,21 @@ __device__ __forceinline__ void group_sums(const Bins &src, int kb, int ke, doub
         w *= rg;
         u += du;
     }
-    group_sum<G>(s);
 }
 
-// F and its derivatives from the window sums (group-uniform).
+// F and its derivatives from the lanes' shares of the window sums: the S_c
+// are summed over the group; the U_c enter only through dF/dtau =
+// sum_c p_c U_c dF/dS_c / tau^2, so each lane folds its shares into one
+// number first and 8 values cross lanes instead of 14.  Group-uniform result.
+template <int G>
 __device__ __forceinline__ Eval eval_from_sums(const double (&s)[2 * NC], double inv_tau) {
     Eval e;
-    const double A = s[0], B = s[1], C = s[2];
-    const double far = s[3], fai = s[4], fbr = s[5], fbi = s[6];
+    double S[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) S[c] = s[c];
+    group_sum<G>(S);
+    const double A = S[0], B = S[1], C = S[2];
+    const double far = S[3], fai = S[4], fbr = S[5], fbi = S[6];
     const double D = fma(A, B, -(C * C));
     const double T = A + B;
     const bool ok = (T > 0.0) && (T * T < QCOND * D);
@@ -202,15 +209,17 @@ __device__ __forceinline__ Eval eval_from_sums(const double (&s)[2 * NC], double
     e.g[4] = 2.0 * (B * fai - C * fbi) * iD;
     e.g[5] = 2.0 * (A * fbr - C * far) * iD;
     e.g[6] = 2.0 * (A * fbi - C * fai) * iD;
-    double d0 = 0.0, d1 = 0.0;
+    double d0 = 0.0;
+    double d1[1] = {0.0};
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const double pc = c < 3 ? 2.0 : 1.0;
-        d0 = fma(pc * s[c], e.g[c], d0);
-        d1 = fma(pc * s[NC + c], e.g[c], d1);
+        d0 = fma(pc * S[c], e.g[c], d0);
+        d1[0] = fma(pc * s[NC + c], e.g[c], d1[0]);
     }
+    group_sum<G>(d1);
     e.dFdt0 = d0 * inv_tau;
-    e.dFdtau = d1 * inv_tau * inv_tau;
+    e.dFdtau = d1[0] * inv_tau * inv_tau;
     return e;
 }
 
@@ -243,7 +252,7 @@ __device__ __forceinline__ Eval eval_group(const Src &sr, const float *lds, int 
         group_sums<G>(LdsBins{(const float4 *)lds, sr.lbase}, kb, ke, t_data, TA, t0, inv_tau, gl, s);
     else
         group_sums<G>(ColBins{sr.g, sr.gstride}, kb, ke, t_data, TA, t0, inv_tau, gl, s);
-    return eval_from_sums(s, inv_tau);
+    return eval_from_sums<G>(s, inv_tau);
 }
 
 // Stage the item's support union into LDS when it fits (bin-major, 8 floats a
@@ -447,8 +456,7 @@ __device__ __forceinline__ Eval eval_regs(const RegSlot<G> &r, double t_data, do
             u += du;
         }
     }
-    group_sum<G>(s);
-    return eval_from_sums(s, inv_tau);
+    return eval_from_sums<G>(s, inv_tau);
 }
 
 // Block blk of a register class (G lanes per candidate of `list`).
