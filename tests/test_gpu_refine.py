@@ -264,6 +264,37 @@ def test_ascent_monotone_bounded_and_converged(small):
     assert close >= 0.8 * prob.n
 
 
+def test_support_class_boundaries(small):
+    """Supports on both sides of each kernel class boundary (1, 8 | 9, 32 |
+    33, ... bins: one lane, four lanes, 16 lanes from LDS), several per
+    template, against the FP64 oracle; the ascent from the same start."""
+    import torch
+
+    from pyfstat_amd.refine import ExpRefineProblem
+
+    work, merged, _ = small
+    rng = np.random.default_rng(7)
+    ct, k_lo, k_hi, t0, tau = [], [], [], [], []
+    for t in range(work.T):
+        N = int(merged.m_off[t + 1] - merged.m_off[t])
+        for L in (1, 2, 7, 8, 9, 16, 31, 32, 33, 64, 100, min(300, N)):
+            lo = int(rng.integers(0, N - L + 1))
+            ct.append(t)
+            k_lo.append(lo)
+            k_hi.append(lo + L - 1)
+            t0.append(float(merged.t_data[t]) + lo * TA + rng.uniform(-0.3, 0.3) * TA)
+            tau.append(max(L, 1) * TA * rng.uniform(0.3, 1.5))
+    prob = ExpRefineProblem(merged, ct, t0, tau, k_lo=k_lo, k_hi=k_hi)
+    L = prob.support_lengths
+    print(f"class boundaries: {prob.n} candidates, lengths {sorted(set(L.tolist()))}", flush=True)
+    _check_forward(prob, merged, work, prob.t0_init.cpu().numpy(), prob.tau_init.cpu().numpy())
+    F0 = prob.forward_raw(merged.cols, prob.t0_init, prob.tau_init, want_saved=False)[0]
+    tau2, twoF, nacc = prob.ascent(merged.cols, n_steps=12)
+    assert torch.all(twoF >= F0 * (1 - 1e-12))
+    again = prob.forward_raw(merged.cols, prob.t0_init, tau2, want_saved=False)[0]
+    np.testing.assert_allclose(again.cpu().numpy(), twoF.cpu().numpy(), rtol=1e-12)
+
+
 def test_global_path_long_support(hip_ready):
     """Supports longer than the LDS staging capacity read atoms from HBM."""
     import torch
