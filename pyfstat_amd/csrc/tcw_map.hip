@@ -1064,6 +1064,9 @@ constexpr int MAX_RB_EXPF = 128;
 // above the map instead of summing the top chunk directly
 constexpr int64_t DIRECT_INIT_MIN_BLOCKS = 256;
 constexpr int INIT_PARTS = 8;  // blocks sharing one column block's direct sum
+// exp_init beside exp_seg on a second stream only when the two launches hold
+// at least this many blocks together (a fork and join costs ~10 µs)
+constexpr int64_t EXP_FORK_MIN_BLOCKS = 1024;
 // Atoms a walk stages in LDS.  Entering (head) atoms are the same for every
 // lane of a row step: FP64 planes of EXPF_HEAD_CAP, read as broadcasts with
 // no conversion.  Leaving (tail) atoms differ per lane (L grows with the
@@ -1774,6 +1777,9 @@ struct tcw_plan {
         d_flags, d_seg_items, d_col_items, d_etab, d_exp_tmpls;
     hipGraphExec_t graph = nullptr;
     hipStream_t cap_stream = nullptr;  // private stream the graph is captured on
+    // exp_init_kernel runs here beside exp_seg_kernel (they are independent)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     const void *g_ts = nullptr, *g_cols = nullptr, *g_F = nullptr, *g_stats = nullptr;
     int64_t g_cstride = -1;
     void reset() {
@@ -1794,6 +1800,9 @@ struct tcw_plan {
     ~tcw_plan() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        if (side) (void)hipStreamDestroy(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         d_tmpl.release();
         d_segs.release();
         d_items.release();
@@ -2124,16 +2133,29 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
                                n_exp, (const SegDev *)p.d_segs.p, ts, cols, cstride,
                                (double *)p.d_merged.p);
     }
-    if (!p.col_items.empty())
-        hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)(p.col_items.size() * INIT_PARTS)),
-                           dim3(WG), 0, st,
-                           (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
-                           (const double *)p.d_merged.p, (double *)p.d_etab.p);
+    // exp_init and exp_seg read the atom table and write different slots of the
+    // E table: with enough blocks for both, exp_init runs on the side stream
+    const int64_t init_blocks = (int64_t)p.col_items.size() * INIT_PARTS;
+    const bool fork = p.side && init_blocks > 0 && !p.seg_items.empty() &&
+                      init_blocks + (int64_t)p.seg_items.size() >= EXP_FORK_MIN_BLOCKS;
+    if (fork) {
+        TCW_HIP_TRY(hipEventRecord(p.ev_fork, st));
+        TCW_HIP_TRY(hipStreamWaitEvent(p.side, p.ev_fork, 0));
+    }
+    if (init_blocks > 0)
+        hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)init_blocks), dim3(WG), 0,
+                           fork ? p.side : st, (const TmplDev *)p.d_tmpl.p,
+                           (const ItemDev *)p.d_col_items.p, (const double *)p.d_merged.p,
+                           (double *)p.d_etab.p);
     if (!p.seg_items.empty())
         hipLaunchKernelGGL(exp_seg_kernel, dim3((unsigned)p.seg_items.size()), dim3(WG),
                            p.max_lds_expf, st, (const TmplDev *)p.d_tmpl.p,
                            (const ItemDev *)p.d_seg_items.p, (const double *)p.d_merged.p,
                            (double *)p.d_etab.p, p.max_lds_expf > 0 ? 1 : 0);
+    if (fork) {
+        TCW_HIP_TRY(hipEventRecord(p.ev_join, p.side));
+        TCW_HIP_TRY(hipStreamWaitEvent(st, p.ev_join, 0));
+    }
     if (!p.col_items.empty())
         hipLaunchKernelGGL(exp_scan_kernel, dim3((unsigned)(p.col_items.size() * SCAN_SPLIT)),
                            dim3(SCAN_WG), 0, st,
@@ -2175,6 +2197,26 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     }
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[3], st));
     if (check_errors) TCW_HIP_TRY(hipGetLastError());
+    return TCW_OK;
+}
+
+// The exp side stream and its events, made on first use (never while a
+// stream is being captured).
+int ensure_side(tcw_plan *p) {
+    if (p->side || p->col_items.empty() || p->seg_items.empty()) return TCW_OK;
+    hipStream_t s = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&a, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (s) (void)hipStreamDestroy(s);
+        if (a) (void)hipEventDestroy(a);
+        return fail(TCW_ERR_DEVICE, std::string("exp side stream: ") + hipGetErrorString(e));
+    }
+    p->side = s;
+    p->ev_fork = a;
+    p->ev_join = b;
     return TCW_OK;
 }
 
@@ -2269,6 +2311,7 @@ int tcw_batch_run(tcw_handle *h, tcw_plan *p, const uint32_t *ts, const float *c
         return fail(TCW_ERR_INVALID, "tcw_batch_run: null argument");
     int rc = set_device(p->device);
     if (rc) return rc;
+    if ((rc = ensure_side(p))) return rc;
     hipStream_t st = (hipStream_t)stream;
     if (!use_graph) return launch_all(*p, ts, cols, col_stride, F_out, stats_dev, st);
     const bool same = p->graph && p->g_ts == ts && p->g_cols == cols && p->g_F == F_out &&
@@ -2317,6 +2360,7 @@ int tcw_batch_run_timed(tcw_handle *h, tcw_plan *p, const uint32_t *ts, const fl
         return fail(TCW_ERR_INVALID, "tcw_batch_run_timed: null argument");
     int rc = set_device(p->device);
     if (rc) return rc;
+    if ((rc = ensure_side(p))) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t ev[4];
     for (int k = 0; k < 4; k++) TCW_HIP_TRY(hipEventCreate(&ev[k]));
