@@ -39,7 +39,9 @@ constexpr int RWG = 256;             // threads per block
 constexpr int LONG_G = 16;           // lanes per candidate in the LDS kernels
 constexpr int CAND_PER_ITEM = RWG / LONG_G;  // candidates per LDS item (one block)
 constexpr int LDS_ATOMS_CAP = 1904;  // staged support bins: 32 B each, 60928 B
-constexpr int BWD_BINS = 256;        // bins per backward block
+constexpr int BWD_WG = 256;          // backward block: BWD_PER bins a thread,
+constexpr int BWD_PER = 4;           // BWD_WG apart (coalesced stores)
+constexpr int BWD_BINS = BWD_WG * BWD_PER;  // bins per backward block
 // Short supports are held in registers, RB bins per lane, loaded once per
 // launch: up to RB bins on one lane per candidate, up to 4*RB bins on four
 // lanes (two DPP steps instead of four), longer ones on the 16-lane groups
@@ -603,34 +605,66 @@ __global__ __launch_bounds__(256) void refine_coef_kernel(
 }
 
 // grad_cols for bins b0 .. b0+nb-1 of one template: sum over the template's
-// candidates of coef_c w_k^{p_c} (FP32; w from an FP64 argument rounded once).
-// Candidate data are uniform loads; candidates whose support misses the block
-// are skipped.
-__global__ __launch_bounds__(BWD_BINS) void refine_backward_kernel(
+// candidates, in order, of coef_c w_k^{p_c} (FP32; w from an FP64 argument
+// rounded once).  The candidates' records are staged in LDS BWD_WG at a time
+// with one coalesced load; a candidate whose support misses the block is
+// skipped (block-uniform test on LDS broadcasts).
+__global__ __launch_bounds__(BWD_WG) void refine_backward_kernel(
     const RTmpl *__restrict__ tm, const BItem *__restrict__ bitems, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const CandCoef *__restrict__ cc, float *__restrict__ grad,
     int64_t cstride) {
+    __shared__ int s_kb[BWD_WG], s_ke[BWD_WG];
+    __shared__ double s_x0[BWD_WG], s_dx[BWD_WG];
+    __shared__ float s_coef[NC][BWD_WG];
     const BItem bi = bitems[blockIdx.x];
     const RTmpl tp = tm[bi.t];
-    const int k = bi.b0 + (int)threadIdx.x;
-    float acc[NC];
+    const int tid = (int)threadIdx.x;
+    float acc[BWD_PER][NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) acc[c] = 0.0f;
+    for (int j = 0; j < BWD_PER; j++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[j][c] = 0.0f;
     const int blo = bi.b0, bhi = bi.b0 + bi.nb - 1;
-    for (int64_t i = tp.cand_beg; i < tp.cand_end; i++) {
-        const int kb = k_lo[i], ke = k_hi[i];
-        if (ke < blo || kb > bhi) continue;  // uniform
-        const CandCoef c = cc[i];
-        if (k >= kb && k <= ke) {
-            const float w = expf((float)-fma((double)k, c.dx, c.x0));
-            const float w2 = w * w;
+    for (int64_t cb = tp.cand_beg; cb < tp.cand_end; cb += BWD_WG) {
+        const int n = (int)min<int64_t>(BWD_WG, tp.cand_end - cb);
+        if (tid < n) {
+            const int64_t i = cb + tid;
+            const CandCoef c = cc[i];
+            s_kb[tid] = k_lo[i];
+            s_ke[tid] = k_hi[i];
+            s_x0[tid] = c.x0;
+            s_dx[tid] = c.dx;
 #pragma unroll
-            for (int q = 0; q < NC; q++) acc[q] = fmaf(c.coef[q], q < 3 ? w2 : w, acc[q]);
+            for (int q = 0; q < NC; q++) s_coef[q][tid] = c.coef[q];
         }
-    }
-    if ((int)threadIdx.x < bi.nb) {
+        __syncthreads();
+        for (int i = 0; i < n; i++) {
+            const int kb = s_kb[i], ke = s_ke[i];
+            if (ke < blo || kb > bhi) continue;  // block-uniform
+            const double x0 = s_x0[i], dx = s_dx[i];
+            float cf[NC];
 #pragma unroll
-        for (int c = 0; c < NC; c++) grad[c * cstride + tp.m_off + k] = acc[c];
+            for (int q = 0; q < NC; q++) cf[q] = s_coef[q][i];
+#pragma unroll
+            for (int j = 0; j < BWD_PER; j++) {
+                const int k = bi.b0 + tid + j * BWD_WG;
+                if (k >= kb && k <= ke) {
+                    const float w = expf((float)-fma((double)k, dx, x0));
+                    const float w2 = w * w;
+#pragma unroll
+                    for (int q = 0; q < NC; q++) acc[j][q] = fmaf(cf[q], q < 3 ? w2 : w, acc[j][q]);
+                }
+            }
+        }
+        __syncthreads();  // before the next chunk overwrites the records
+    }
+#pragma unroll
+    for (int j = 0; j < BWD_PER; j++) {
+        const int r = tid + j * BWD_WG;
+        if (r < bi.nb) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) grad[c * cstride + tp.m_off + bi.b0 + r] = acc[j][c];
+        }
     }
 }
 
@@ -882,7 +916,7 @@ int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau,
                            g_lnBtSG, grad_t0, grad_tau,
                            grad_cols ? (CandCoef *)p->d_coef.p : nullptr);
     if (grad_cols)
-        hipLaunchKernelGGL(refine_backward_kernel, dim3((unsigned)p->n_bitems), dim3(BWD_BINS), 0, st,
+        hipLaunchKernelGGL(refine_backward_kernel, dim3((unsigned)p->n_bitems), dim3(BWD_WG), 0, st,
                            (const RTmpl *)p->d_tmpl.p, (const BItem *)p->d_bitems.p,
                            (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p,
                            (const CandCoef *)p->d_coef.p, grad_cols, col_stride);

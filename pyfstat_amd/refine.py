@@ -297,8 +297,12 @@ class ExpRefineProblem:
                      want_tau=True, cols_shape=None):
         torch = _torch()
         ncols = cols_shape[1] if cols_shape is not None else self.merged.total_bins
-        gc = (torch.zeros((7, ncols), dtype=torch.float32, device=self.device)
-              if want_cols else None)
+        gc = None
+        if want_cols:
+            # the kernel writes every merged bin; only columns past them need zeros
+            gc = torch.empty((7, ncols), dtype=torch.float32, device=self.device)
+            if ncols > self.merged.total_bins:
+                gc[:, self.merged.total_bins:].zero_()
         gt0 = torch.empty(self.n, dtype=torch.float64, device=self.device) if want_t0 else None
         gtau = torch.empty(self.n, dtype=torch.float64, device=self.device) if want_tau else None
         g2 = g_twoF.to(torch.float64).contiguous() if g_twoF is not None else None

@@ -209,6 +209,24 @@ def test_autograd_backward_vs_oracle(small):
     assert cols_h.shape == got.shape
 
 
+def test_backward_wider_cols(small):
+    """cols with columns past the merged bins: their gradient is zero and the
+    rest equals the gradient for the exact-width cols."""
+    import torch
+
+    work, merged, _ = small
+    prob, _ = _problem(work, merged)
+    nb = merged.total_bins
+    wide = torch.full((7, nb + 37), 123.0, dtype=torch.float32, device=_device())
+    wide[:, :nb] = merged.cols
+    cw = wide.clone().requires_grad_(True)
+    prob(cw, prob.t0_init, prob.tau_init)[0].sum().backward()
+    c = merged.cols.clone().requires_grad_(True)
+    prob(c, prob.t0_init, prob.tau_init)[0].sum().backward()
+    assert torch.equal(cw.grad[:, :nb], c.grad)
+    assert torch.all(cw.grad[:, nb:] == 0)
+
+
 def test_raw_atom_gradients_through_merge(small):
     work, merged, cols_d = small
     prob, _ = _problem(work, merged)
