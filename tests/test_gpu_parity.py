@@ -352,6 +352,38 @@ def test_batched_matches_single(hip_ready):
     b.close()
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batched_exp_direct_sum_side_stream(hip_ready, use_graph):
+    """Exp batch with enough (template, column block) pairs for the direct
+    top-chunk sum and enough workgroups that it runs on the plan's second
+    stream beside the chunk walks: every map agrees with the single-map call
+    (which takes neither), a sample with the oracle; graph replays and the
+    eager run give the same bits."""
+    from pyfstat_amd import tcw_fstat_map_funcs as tcw
+    from pyfstat_amd.batched import BatchedTransientMaps
+
+    multis, win = synth.workload_exp_shared(T=288, n_sft=700, n_t0=256, n_tau=64, seed_index=44)
+    b = BatchedTransientMaps.from_multi_atoms(multis, [win] * len(multis), BtSG=True)
+    b.run(use_graph=False)
+    ref = [m.F_mn.copy() for m in b.maps()]
+    for rep in range(3 if use_graph else 1):
+        b.F_d.fill_(-1.0)  # eager work between replays
+        b.run(use_graph=use_graph)
+        maps = b.maps()
+        for t in range(len(multis)):
+            assert np.array_equal(maps[t].F_mn, ref[t]), (rep, t)
+    for t in range(len(multis)):
+        fm = tcw.hip_compute_transient_fstat_map(multis[t], win, BtSG=True)
+        np.testing.assert_allclose(maps[t].F_mn, fm.F_mn, rtol=2e-6, atol=0)
+        assert (maps[t].t0_ML, maps[t].tau_ML) == (fm.t0_ML, fm.tau_ML)
+    for t in (0, 97, 287):
+        print(f"exp side-stream batch template {t}", flush=True)
+        om = oracle_for(multis[t], win)
+        compare_maps(maps[t].F_mn, om)
+        assert abs(maps[t].lnBtSG - om.lnBtSG) <= lnb_tol(om.maxF)
+    b.close()
+
+
 def test_batched_mixed_windows_multi_detector(hip_ready):
     """Ragged batch mixing rect (regular and generic) and exp windows, 1-3 detectors."""
     from pyfstat_amd.batched import hip_compute_transient_fstat_maps
