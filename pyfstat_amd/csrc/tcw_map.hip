@@ -289,14 +289,22 @@ __global__ __launch_bounds__(SR_WG) void stage_kernel(
 // workgroups (one large map would otherwise occupy a handful of CUs).
 constexpr int SX_NARROW_BELOW = 256;
 constexpr int SX_CHUNK = 1024;  // atoms staged per pass
+// the finished rows are restaged in LDS (row stride SX_ROW floats, odd so the
+// lanes' writes spread over the banks) and written to the FP64 table with
+// consecutive lanes on consecutive doubles
+constexpr int SX_ROW = MERGED_STRIDE + 1;
 
 template <int SXB>
 __global__ __launch_bounds__(256) void stage_exp_kernel(
     const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ ts,
     const float *__restrict__ cols, int64_t cstride, double *__restrict__ merged) {
-    __shared__ uint32_t s_ts[SX_CHUNK];
-    __shared__ float s_x[NC][SX_CHUNK];
+    // atom staging during the merge, the finished rows afterwards
+    constexpr int SX_RAW = (NC + 1) * SX_CHUNK > 256 * SXB * SX_ROW ? (NC + 1) * SX_CHUNK
+                                                                     : 256 * SXB * SX_ROW;
+    __shared__ float s_raw[SX_RAW];
+    uint32_t *s_ts = reinterpret_cast<uint32_t *>(s_raw);
+    float(*s_x)[SX_CHUNK] = reinterpret_cast<float(*)[SX_CHUNK]>(s_raw + SX_CHUNK);
     __shared__ int64_t s_rng[2];
     const int tid = threadIdx.x;
     for (int64_t e = blockIdx.y; e < n_exp; e += gridDim.y) {
@@ -363,15 +371,20 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
                 __syncthreads();
             }
         }
+        // the last staging pass ended with a barrier: s_raw is free
 #pragma unroll
         for (int j = 0; j < SX_BINS; j++) {
-            if (b0 + j < B1) {
-                double *row = merged + td.merged_off + (int64_t)(b0 + j) * MERGED_STRIDE;
+            float *r = s_raw + (tid * SX_BINS + j) * SX_ROW;
 #pragma unroll
-                for (int c = 0; c < NC; c++) row[c] = (double)v[j][c];
-                row[NC] = 0.0;
-            }
+            for (int c = 0; c < NC; c++) r[c] = v[j][c];
+            r[NC] = 0.0f;
         }
+        __syncthreads();
+        double *base = merged + td.merged_off + (int64_t)B0 * MERGED_STRIDE;
+        const int nq = (int)(B1 - B0) * MERGED_STRIDE;
+        for (int q = tid; q < nq; q += 256)
+            base[q] = (double)s_raw[(q / MERGED_STRIDE) * SX_ROW + q % MERGED_STRIDE];
+        __syncthreads();  // s_raw is staged again for the next template
     }
 }
 
