@@ -313,6 +313,31 @@ def test_support_class_boundaries(small):
     np.testing.assert_allclose(again.cpu().numpy(), twoF.cpu().numpy(), rtol=1e-12)
 
 
+def test_ascent_nonpositive_tau(small):
+    """A start tau <= 0 (or NaN) gives 2F = NaN, tau passed through and no
+    accepted step, in every kernel class; the other candidates are unchanged."""
+    import torch
+
+    work, merged, _ = small
+    prob, _ = _problem(work, merged)
+    L = prob.support_lengths
+    pick = [int(np.nonzero(L <= 8)[0][0]), int(np.nonzero((L > 8) & (L <= 32))[0][0]),
+            int(np.nonzero(L > 32)[0][0])]
+    tau = prob.tau_init.clone()
+    bad = torch.tensor([0.0, -5.0, float("nan")], dtype=torch.float64, device=tau.device)
+    tau[pick] = bad
+    tau_out, twoF, nacc = prob.ascent(merged.cols, tau=tau, n_steps=8)
+    ref_tau, ref_twoF, ref_nacc = prob.ascent(merged.cols, n_steps=8)
+    keep = np.ones(prob.n, bool)
+    keep[pick] = False
+    assert torch.all(torch.isnan(twoF[pick]))
+    assert torch.equal(nacc[pick], torch.zeros(3, dtype=torch.int32, device=nacc.device))
+    got = tau_out[pick].cpu().numpy()
+    assert got[0] == 0.0 and got[1] == -5.0 and np.isnan(got[2])
+    k = torch.from_numpy(keep).to(twoF.device)
+    assert torch.equal(twoF[k], ref_twoF[k]) and torch.equal(tau_out[k], ref_tau[k])
+
+
 def test_global_path_long_support(hip_ready):
     """Supports longer than the LDS staging capacity read atoms from HBM."""
     import torch
