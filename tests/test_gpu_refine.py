@@ -359,6 +359,32 @@ def test_global_path_long_support(hip_ready):
                                               want_saved=False)[0] * (1 - 1e-12))
 
 
+def test_wide_lds_unions(hip_ready):
+    """Support unions of ~1200 bins (two long candidates per template, well
+    inside the LDS staging capacity): forward and ascent against the
+    reference."""
+    import torch
+
+    from pyfstat_amd.refine import ExpRefineProblem
+
+    work = synth.workload_exp_refine(T=2, n_sft=3000, n_t0=8, n_tau=8, seed_index=42)
+    merged, _ = _merged(work)
+    ct = np.array([0, 0, 1, 1])
+    k_lo = np.array([0, 500, 100, 150])
+    k_hi = np.array([600, 1199, 1250, 1300])
+    t0 = np.array([float(merged.t_data[t]) + k * TA for t, k in zip(ct, k_lo)])
+    tau = np.array([200.0, 250.0, 380.0, 300.0]) * TA
+    prob = ExpRefineProblem(merged, ct, t0, tau, k_lo=k_lo, k_hi=k_hi)
+    print(f"wide LDS unions: supports {prob.support_lengths.tolist()}", flush=True)
+    _check_forward(prob, merged, work, prob.t0_init.cpu().numpy(),
+                   prob.tau_init.cpu().numpy() * 1.01)
+    F0 = prob.forward_raw(merged.cols, prob.t0_init, prob.tau_init, want_saved=False)[0]
+    tau2, twoF, _ = prob.ascent(merged.cols, n_steps=6)
+    assert torch.all(twoF >= F0 * (1 - 1e-12))
+    again = prob.forward_raw(merged.cols, prob.t0_init, tau2, want_saved=False)[0]
+    np.testing.assert_allclose(again.cpu().numpy(), twoF.cpu().numpy(), rtol=1e-12)
+
+
 def test_problem_validation(small):
     from pyfstat_amd.refine import ExpRefineProblem
 
