@@ -254,10 +254,7 @@ int tcw_refine_plan_destroy(tcw_refine_plan *p);
  * saved[i*TCW_REFINE_SAVED + j] = dF/dt0, dF/dtau, dF/dA, dF/dB, dF/dC,
  * dF/dFa_re, dF/dFa_im, dF/dFb_re, dF/dFb_im, F (may be NULL).
  * cols: device merged atoms [TCW_NCOL][col_stride] (tcw_merge_batch layout);
- * t0, tau: device double [n_cand].  Asynchronous on `stream`: short supports
- * run on a stream of the plan that starts after the work queued on `stream`
- * and that `stream` waits for before its later work, so forward and ascent
- * calls on one plan must not overlap (from several host threads). */
+ * t0, tau: device double [n_cand].  Asynchronous on `stream`. */
 int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride,
                        const double *t0, const double *tau, double *twoF, double *lnBtSG,
                        double *saved, void *stream);
@@ -284,8 +281,10 @@ int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau,
  * (eta *= 0.5); the first step moves s by step0.  t0 is not moved (dF/dt0 = 0
  * on a fixed support).  Starts from tau_in and writes the final tau to
  * tau_out (the two may be the same array); twoF receives the final 2F
- * and n_accept (may be NULL) the accepted steps.  Asynchronous on `stream`,
- * as tcw_refine_forward (same side stream). */
+ * and n_accept (may be NULL) the accepted steps.  Asynchronous on `stream`:
+ * short supports run on a stream of the plan that starts after the work
+ * queued on `stream` and that `stream` waits for before its later work, so
+ * ascent calls on one plan must not overlap (from several host threads). */
 int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
                       const double *t0, const double *tau_in, double *tau_out,
                       const double *tau_lo, const double *tau_hi, int n_steps, double step0,

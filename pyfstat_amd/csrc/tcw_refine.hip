@@ -880,8 +880,8 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
     if (col_stride < p->total_bins) return fail(TCW_ERR_INVALID, "col_stride smaller than the merged bins");
     const int32_t *order = (const int32_t *)p->d_order.p;
     const hipStream_t st = (hipStream_t)stream;
-    hipStream_t side;
-    if ((rc = fork_side(p, st, &side))) return rc;
+    // one evaluation: both launches on st (a fork and join to the side stream
+    // costs more than their overlap saves)
     if (p->n_items > 0)
         hipLaunchKernelGGL(refine_forward_kernel<LONG_G>, dim3((unsigned)p->n_items),
                            dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
@@ -892,12 +892,12 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
     const unsigned mid_blocks = reg_blocks(p->n_mid, 4), short_blocks = reg_blocks(n_short, 1);
     if (mid_blocks + short_blocks > 0)
         hipLaunchKernelGGL(refine_forward_regs_kernel, dim3(mid_blocks + short_blocks), dim3(RWG), 0,
-                           side, (const RTmpl *)p->d_tmpl.p, (const int32_t *)p->d_ctmpl.p,
+                           st, (const RTmpl *)p->d_tmpl.p, (const int32_t *)p->d_ctmpl.p,
                            order + p->n_long, p->n_mid, mid_blocks, order + p->n_long + p->n_mid,
                            n_short, (const int32_t *)p->d_klo.p, (const int32_t *)p->d_khi.p, cols,
                            col_stride, (double)p->TAtom, t0, tau, twoF, lnBtSG, saved);
     TCW_HIP_TRY(hipGetLastError());
-    return join_side(p, st, side);
+    return TCW_OK;
 }
 
 int tcw_refine_backward(tcw_refine_plan *p, const double *t0, const double *tau, const double *saved,
