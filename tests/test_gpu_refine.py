@@ -282,6 +282,30 @@ def test_ascent_monotone_bounded_and_converged(small):
     assert close >= 0.8 * prob.n
 
 
+def test_ascent_on_user_stream(small):
+    """The ascent forks part of its work to the plan's own stream: on a torch
+    stream of the caller's, it still starts after the work queued there (the
+    start tau is written on that stream just before) and the caller's later
+    work sees its results."""
+    import torch
+
+    work, merged, _ = small
+    prob, _ = _problem(work, merged)
+    ref_tau, ref_twoF, ref_nacc = prob.ascent(merged.cols, n_steps=10)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        start = torch.empty_like(prob.tau_init)
+        for _ in range(20):  # queued work the ascent must wait for
+            start.copy_(prob.tau_init * 0.5)
+            start.mul_(2.0)
+        tau, twoF, nacc = prob.ascent(merged.cols, tau=start, n_steps=10)
+        twoF_after = twoF * 1.0  # later work on the same stream
+    s.synchronize()
+    assert torch.equal(tau, ref_tau) and torch.equal(twoF, ref_twoF)
+    assert torch.equal(nacc, ref_nacc) and torch.equal(twoF_after, ref_twoF)
+
+
 def test_support_class_boundaries(small):
     """Supports on both sides of each kernel class boundary (1, 8 | 9, 32 |
     33, ... bins: one lane, four lanes, 16 lanes from LDS), several per
