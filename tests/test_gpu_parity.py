@@ -381,6 +381,21 @@ def test_batched_exp_direct_sum_side_stream(hip_ready, use_graph):
         om = oracle_for(multis[t], win)
         compare_maps(maps[t].F_mn, om)
         assert abs(maps[t].lnBtSG - om.lnBtSG) <= lnb_tol(om.maxF)
+    # on a caller's stream right after queued work that doubles every atom:
+    # the fork to the plan's second stream waits for it, so F doubles exactly
+    # (power-of-two scaling commutes with every rounding; F = 2 fallback
+    # cells stay 2)
+    import torch
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.cols_d.mul_(2.0)
+        b.run(use_graph=use_graph, stream=s)
+    s.synchronize()
+    maps2 = b.maps()
+    for t in range(len(multis)):
+        want = np.where(ref[t] == 2.0, np.float32(2.0), np.float32(2.0) * ref[t])
+        assert np.array_equal(maps2[t].F_mn, want), t
     b.close()
 
 
