@@ -56,6 +56,7 @@ enum Mode : int32_t {
 };
 
 constexpr int RB_DEFAULT[MODE_COUNT] = {128, 16, 16, 128};
+constexpr int64_t SMALL_BATCH_ITEMS = 512;  // rect items aimed at when rows are not tuned
 
 struct TmplDev {
     int64_t seg_begin, seg_end;  // detector segments of this template
@@ -1946,6 +1947,14 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
     }
     td.mode = mode;
     int rb = g_rb_override[mode] > 0 ? g_rb_override[mode] : RB_DEFAULT[mode];
+    if (mode == MODE_RECT_FAST && g_rb_override[mode] <= 0) {
+        // few or small maps (the single-map call): fewer rows per item, so
+        // that the batch's items cover the CUs (about SMALL_BATCH_ITEMS of them
+        // if the templates are alike)
+        const int64_t ncb = (N_tau + TILE_COLS - 1) / TILE_COLS;
+        const int64_t want = (N_t0 * ncb * p.T + SMALL_BATCH_ITEMS - 1) / SMALL_BATCH_ITEMS;
+        rb = (int)std::min<int64_t>(RB_DEFAULT[mode], std::max<int64_t>(16, (want + 15) / 16 * 16));
+    }
     td.rb_rows = rb;
     td.n_rowblk = (uint32_t)((N_t0 + rb - 1) / rb);
     td.n_colblk = (uint32_t)((N_tau + TILE_COLS - 1) / TILE_COLS);
