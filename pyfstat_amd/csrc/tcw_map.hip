@@ -57,6 +57,7 @@ enum Mode : int32_t {
 
 constexpr int RB_DEFAULT[MODE_COUNT] = {128, 16, 16, 128};
 constexpr int64_t SMALL_BATCH_ITEMS = 512;  // rect items aimed at when rows are not tuned
+constexpr int EXP_SMALL_BATCH_ROWS = 64;    // exp rows when untuned and under 256 column blocks
 
 struct TmplDev {
     int64_t seg_begin, seg_end;  // detector segments of this template
@@ -1955,6 +1956,9 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         const int64_t want = (N_t0 * ncb * p.T + SMALL_BATCH_ITEMS - 1) / SMALL_BATCH_ITEMS;
         rb = (int)std::min<int64_t>(RB_DEFAULT[mode], std::max<int64_t>(16, (want + 15) / 16 * 16));
     }
+    if (mode == MODE_EXP_FAST && g_rb_override[mode] <= 0 &&
+        p.T * ((N_tau + TILE_COLS - 1) / TILE_COLS) < DIRECT_INIT_MIN_BLOCKS)
+        rb = EXP_SMALL_BATCH_ROWS;  // few column blocks: shorter chunks, more of them
     td.rb_rows = rb;
     td.n_rowblk = (uint32_t)((N_t0 + rb - 1) / rb);
     td.n_colblk = (uint32_t)((N_tau + TILE_COLS - 1) / TILE_COLS);
