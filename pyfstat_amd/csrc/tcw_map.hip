@@ -1952,9 +1952,10 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         // few or small maps (the single-map call): fewer rows per item, so
         // that the batch's items cover the CUs (about SMALL_BATCH_ITEMS of them
         // if the templates are alike)
-        const int64_t ncb = (N_tau + TILE_COLS - 1) / TILE_COLS;
-        const int64_t want = (N_t0 * ncb * p.T + SMALL_BATCH_ITEMS - 1) / SMALL_BATCH_ITEMS;
-        rb = (int)std::min<int64_t>(RB_DEFAULT[mode], std::max<int64_t>(16, (want + 15) / 16 * 16));
+        const double ncb = (double)((N_tau + TILE_COLS - 1) / TILE_COLS);
+        const double want = (double)N_t0 * ncb * (double)p.T / (double)SMALL_BATCH_ITEMS;
+        rb = want >= RB_DEFAULT[mode] ? RB_DEFAULT[mode]
+                                      : std::max(16, ((int)std::ceil(want) + 15) / 16 * 16);
     }
     if (mode == MODE_EXP_FAST && g_rb_override[mode] <= 0 &&
         p.T * ((N_tau + TILE_COLS - 1) / TILE_COLS) < DIRECT_INIT_MIN_BLOCKS)
