@@ -254,7 +254,7 @@ int tcw_refine_plan_destroy(tcw_refine_plan *p);
  * saved[i*TCW_REFINE_SAVED + j] = dF/dt0, dF/dtau, dF/dA, dF/dB, dF/dC,
  * dF/dFa_re, dF/dFa_im, dF/dFb_re, dF/dFb_im, F (may be NULL).
  * cols: device merged atoms [TCW_NCOL][col_stride] (tcw_merge_batch layout);
- * t0, tau: device double [n_cand].  Asynchronous on `stream`. */
+ * t0, tau: device double [n_cand], tau > 0.  Asynchronous on `stream`. */
 int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride,
                        const double *t0, const double *tau, double *twoF, double *lnBtSG,
                        double *saved, void *stream);
