@@ -131,7 +131,9 @@ constexpr int SR_PASS_BINS = SR_PASS_LANES * SR_BINS;  // 128 bins per wave per 
 __global__ __launch_bounds__(SR_WG) void stage_kernel(
     const TmplDev *__restrict__ tds, const SegDev *__restrict__ segs,
     const uint32_t *__restrict__ ts, const float *__restrict__ cols, int64_t cstride,
-    double *__restrict__ prefix) {
+    double *__restrict__ prefix, int *__restrict__ flags) {
+    // one block per template, before any map kernel: clear its single-atom flag
+    if (threadIdx.x == 0) flags[blockIdx.x] = 0;
     const TmplDev td = tds[blockIdx.x];
     if (td.mode != MODE_RECT_FAST && td.mode != MODE_RECT_GENERIC) return;  // block-uniform
     // staging: ts [SR_CHUNK] uint32, then NC float planes [SR_CHUNK];
@@ -1754,11 +1756,6 @@ __global__ __launch_bounds__(256) void gather_bins_kernel(const float *__restric
     }
 }
 
-__global__ void zero_ints_kernel(int *p, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 // ----------------------------------------------------------------------------
 // host side
 // ----------------------------------------------------------------------------
@@ -2139,12 +2136,10 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     const bool btsg = p.flags & TCW_FLAG_BTSG;
     const bool strict = p.flags & TCW_FLAG_LAL_STRICT;
     auto *flags = (int *)p.d_flags.p;
-    hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)((p.T + 255) / 256)), dim3(256), 0, st,
-                       flags, p.T);
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[0], st));
     hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(SR_WG), 0, st,
                        (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
-                       cstride, (double *)p.d_prefix.p);
+                       cstride, (double *)p.d_prefix.p, flags);
     if (!p.exp_tmpls.empty()) {
         const int64_t n_exp = (int64_t)p.exp_tmpls.size();
         const int64_t wide_blocks = (p.exp_max_N + 1023) / 1024;
