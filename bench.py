@@ -5,7 +5,7 @@ Default workload: rectangular-window maps for 4096 templates with ragged
 N_sft ~ U[1000, 8000] (10% random gaps each, 5% with an injected transient),
 N_t0 = N_tau = N_sft // 4, lnBtSG and the posterior modes fused
 (BtSG=True) -- 6.3e9 2F cells per step.  One step = the whole batch through
-the device path (binned merge + compensated prefix, map kernels with fused
+the device path (binned merge + FP64 prefix scan, map kernels with fused
 reductions, per-template finalize), replayed from a hipGraph; the atoms are
 resident in HBM before the timed region.
 

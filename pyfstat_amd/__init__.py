@@ -9,11 +9,12 @@ Modules:
   tcw_fstat_map_funcs  drop-in counterpart of pyfstat.tcw_fstat_map_funcs
   batched              many templates in one device pass
   atoms                atom and window-range containers (lalpulsar stand-ins)
+  windows              window ranges from ComputeFstat-style options
   synth                seeded synthetic atoms for the benchmark workloads
   build                in-tree hipcc build of the library
 """
 
-from . import atoms, synth
+from . import atoms, synth, windows
 from .atoms import (
     TRANSIENT_EXPONENTIAL,
     TRANSIENT_LAST,
@@ -31,6 +32,7 @@ __version__ = "0.1.0"
 __all__ = [
     "atoms",
     "synth",
+    "windows",
     "TRANSIENT_NONE",
     "TRANSIENT_RECTANGULAR",
     "TRANSIENT_EXPONENTIAL",

@@ -2,9 +2,10 @@
 //
 // Device path for one or many templates:
 //   1. stage_kernel    binned detector merge (lalpulsar mergeMultiFstatAtomsBinned,
-//                      called at pyfstat/tcw_fstat_map_funcs.py:706) fused with a
-//                      compensated FP32 (two-sum) prefix scan of the 7 atom
-//                      columns (rect windows) or an FP64 atom table (exp windows);
+//                      called at pyfstat/tcw_fstat_map_funcs.py:706) fused with an
+//                      FP64 prefix scan of the float32 merged bins (rect windows);
+//                      stage_exp_kernel: the same merge into an FP64 atom table
+//                      (exp windows);
 //   2. map kernels     window sums -> cond-guarded F per (t0, tau) cell, with the
 //                      max/argmax, logsumexp and row/column posterior sums fused
 //                      into the producer (pyfstat/tcw_fstat_map_funcs.py:809-828);
