@@ -144,94 +144,271 @@ def plain_loop(b, batch, idx, BtSG):
     return cells, time.perf_counter() - t0
 
 
-def cpu_baseline(batch, budget_s, seed=5):
-    """FP64 C oracle, OpenMP over templates, on a seeded sample of templates
-    sized to about budget_s seconds of all-core CPU time."""
+class Units:
+    """Independent pieces of a workload that the FP64 C oracle maps on the
+    CPU: unit i -> (merged atoms float32 [7, N], t0_data, window, cells).
+
+    rect_batched: whole templates (seeded order); rect_plain: the one template
+    (the all-core figure runs one copy per core, as one lal process per core
+    would); exp_plain / exp_shared: single t0 rows (1-row windows over the full
+    tau range) of seeded-random (template, row) pairs -- the direct exp sums are
+    too slow on the CPU for whole maps, so the rate is measured on rows and
+    applies to the whole map (every row costs about the same on average).
+    """
+
+    def __init__(self, args, work, batch, seed=5):
+        import oracle
+        from pyfstat_amd.atoms import atoms_to_arrays, transientWindowRange_t
+
+        self.kind = args.workload
+        self._oracle = oracle
+        self._Win = transientWindowRange_t
+        self._merged = {}
+        rng = np.random.default_rng(seed)
+        if batch is not None:
+            self.templates = batch.T
+            self.windows = batch.windows
+            self._atoms = lambda t: (batch.ts[batch.off[t]:batch.off[t + 1]],
+                                     batch.cols[:, batch.off[t]:batch.off[t + 1]],
+                                     np.array([0, batch.off[t + 1] - batch.off[t]]), batch.TAtom)
+        else:
+            multis, wins = work
+            self.templates = len(multis)
+            self.windows = wins
+            packed = [None] * len(multis)
+
+            def atoms(t):
+                if packed[t] is None:
+                    packed[t] = atoms_to_arrays(multis[t])
+                p = packed[t]
+                return p.ts, p.cols, p.det_off, p.TAtom
+            self._atoms = atoms
+        self.rows = self.kind in ("exp_plain", "exp_shared")
+        if self.rows:
+            n_t0 = [w.t0Band // w.dt0 + 1 for w in self.windows]
+            pairs = [(t, m) for t in range(self.templates) for m in range(n_t0[t])]
+            self.order = [pairs[i] for i in rng.permutation(len(pairs))]
+        elif self.kind == "rect_plain":
+            self.order = None
+        else:
+            self.order = [(int(t), None) for t in rng.permutation(self.templates)]
+        self.n = None if self.order is None else len(self.order)
+        self.with_lnBtSG = not self.rows
+
+    def merged(self, t):
+        if t not in self._merged:
+            ts, cols, det_off, TA = self._atoms(t)
+            self._merged[t] = (self._oracle.merge_binned(ts, cols, det_off, TA), TA)
+        return self._merged[t]
+
+    def __call__(self, i):
+        t, m = (0, None) if self.order is None else self.order[i]
+        (merged, t0d), TA = self.merged(t)
+        w = self.windows[t]
+        if m is not None:
+            w = self._Win(w.type, w.t0 + m * w.dt0, 0, w.dt0, w.tau, w.tauBand, w.dtau)
+        cells = (w.t0Band // w.dt0 + 1) * (w.tauBand // w.dtau + 1)
+        return merged, t0d, w, TA, cells, (t, m)
+
+    def describe(self, k):
+        if self.rows:
+            return f"{k} seeded-random t0 rows (1-row windows, full tau range)"
+        if self.order is None:
+            return f"{k} copies of the template"
+        return f"{k} seeded-random templates"
+
+
+# Cells one oracle call holds on the host (8 bytes each): samples larger than
+# this are timed by repeating a capped sample instead.
+CPU_CELL_CAP = 2.5e8
+
+
+def _time_units(units, k, threads, reps=1):
+    """Map units 0..k-1 on `threads` OpenMP threads, reps times; (cells, seconds)."""
     import oracle
 
+    oracle.set_threads(threads)
+    sample = [units(i) for i in range(k)]  # merged atoms prepared outside the clock
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.compute_map_batch([x[0] for x in sample], [x[3] for x in sample],
+                                 [x[1] for x in sample], [x[2] for x in sample],
+                                 with_lnBtSG=units.with_lnBtSG)
+    dt = time.perf_counter() - t0
+    return reps * sum(x[4] for x in sample), dt
+
+
+def _sized_run(units, threads, budget_s):
+    """Grow the sample until one timed run takes about budget_s; past
+    CPU_CELL_CAP cells, repeat the capped sample instead.  Returns
+    (units in the sample, repetitions, cells, seconds)."""
+    cap = units.n if units.n is not None else 10**9
+    k = min(threads, cap)
+    cells, dt = _time_units(units, k, threads)
+    for _ in range(4):
+        if dt >= 0.7 * budget_s or k >= cap:
+            break
+        kmax = max(threads, int(CPU_CELL_CAP / max(cells / k, 1.0)))
+        k_new = min(cap, kmax, max(k + 1, int(k * budget_s / max(dt, 1e-3)) + 1))
+        if k_new <= k:
+            break
+        k = k_new
+        cells, dt = _time_units(units, k, threads)
+    reps = 1
+    if dt < 0.7 * budget_s:
+        reps = int(np.ceil(budget_s / max(dt, 1e-3)))
+        cells, dt = _time_units(units, k, threads, reps)
+    return k, reps, cells, dt
+
+
+def cpu_baseline(units, budget_s):
+    """FP64 C oracle (built here with -O3 -march=native), single core and all
+    cores (OpenMP over units), on bounded seeded samples of the workload."""
+    import oracle
+
+    flags = oracle.select_native()
     cores = len(os.sched_getaffinity(0))
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp:
         cores = min(cores, int(omp))
-    order = np.random.default_rng(seed).permutation(batch.T)
-
-    def prepare(ts_idx):
-        out = []
-        for t in ts_idx:
-            a, e = int(batch.off[t]), int(batch.off[t + 1])
-            m, t0d = oracle.merge_binned(batch.ts[a:e], batch.cols[:, a:e],
-                                         np.array([0, e - a]), batch.TAtom)
-            w = batch.windows[t]
-            out.append((m, t0d, w, (w.t0Band // w.dt0 + 1) * (w.tauBand // w.dtau + 1)))
-        return out
-
-    def run(sample):
-        t0 = time.perf_counter()
-        oracle.compute_map_batch([x[0] for x in sample], [batch.TAtom] * len(sample),
-                                 [x[1] for x in sample], [x[2] for x in sample],
-                                 with_lnBtSG=True)
-        return time.perf_counter() - t0
-
-    sample = prepare(order[:cores])  # calibration: one template per core
-    dt = run(sample)
-    rate = sum(x[3] for x in sample) / dt
-    want = rate * budget_s
-    k = cores
-    total = sum(x[3] for x in sample)
-    all_cells = batch.cells()
-    while k < batch.T and total < want:
-        k += 1
-        total += int(all_cells[order[k - 1]])
-    if k > cores:
-        sample = prepare(order[:k])
-        dt = run(sample)
-    # OpenMP scaling on the calibration sample misjudges the rate: grow the
-    # sample until it takes about the budget
-    for _ in range(3):
-        if dt >= 0.8 * budget_s or k >= batch.T:
-            break
-        k = min(batch.T, int(k * budget_s / max(dt, 1e-3)) + 1)
-        sample = prepare(order[:k])
-        dt = run(sample)
-    ncells = sum(x[3] for x in sample)
+    k1, r1, c1, t1 = _sized_run(units, 1, 0.3 * budget_s)
+    log(f"cpu single core: {units.describe(k1)} x {r1}, {c1:.3e} cells in {t1:.2f} s")
+    kn, rn, cn, tn = _sized_run(units, cores, 0.7 * budget_s)
+    log(f"cpu {cores} cores: {units.describe(kn)} x {rn}, {cn:.3e} cells in {tn:.2f} s")
+    frac = ""
+    if units.rows:
+        total_rows = units.n
+        frac = f"; {kn}/{total_rows} = {kn / total_rows:.2%} of the workload's rows"
+    if rn > 1:
+        frac += f"; the sample mapped {rn} times"
     return {
-        "value": ncells / dt,
+        "value": cn / tn,
         "unit": "cells/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{len(sample)} seeded-random templates of the workload ({ncells:.3e} "
-                  f"cells, {dt:.1f} s), FP64 C oracle with lnBtSG, OpenMP over templates",
+        "single_core": c1 / t1,
+        "cpu_model": oracle.cpu_model(),
+        "build": flags,
+        "sample": f"{units.describe(kn)} ({cn:.3e} cells, {tn:.1f} s{frac}), FP64 C oracle "
+                  f"(lal-equivalent restatement, not lalpulsar)"
+                  f"{' with lnBtSG' if units.with_lnBtSG else ''}, OpenMP over the sample; "
+                  f"single core: {units.describe(k1)}{f' x {r1}' if r1 > 1 else ''} "
+                  f"({c1:.3e} cells, {t1:.1f} s)",
     }
 
 
-def parity_sample(b, batch, n, seed=3):
+def _err_figures(F_gpu, om_F, om_cond):
+    mask = np.abs(om_cond / 1e4 - 1) < 1e-3
+    rel = np.abs(F_gpu - om_F) / np.maximum(np.abs(om_F), 1e-30)
+    return float(np.where(mask, 0, rel).max()), int(mask.sum())
+
+
+def parity_sample(b, units, n, seed=3):
+    """GPU maps against the FP64 oracle on a seeded sample.
+
+    Whole templates (rect workloads): every cell, the fused argmax and lnBtSG.
+    Rows (exp workloads, 1-row oracle windows): every cell of each row; the
+    fused argmax and lnBtSG are then checked against the GPU map itself
+    (argmax of the stored map, float64 logsumexp of it), since a full FP64
+    direct-sum map of these sizes takes minutes on the CPU.
+    """
     import oracle
 
     if n <= 0:
         return None
-    rng = np.random.default_rng(seed)
-    idx = sorted(rng.choice(batch.T, size=min(n, batch.T), replace=False).tolist())
     st = b.stats()
-    worst, worst_lnb, argmax_exact, near = 0.0, 0.0, 0, 0
-    for t in idx:
-        a, e = int(batch.off[t]), int(batch.off[t + 1])
-        m, t0d = oracle.merge_binned(batch.ts[a:e], batch.cols[:, a:e], np.array([0, e - a]),
-                                     batch.TAtom)
-        om = oracle.compute_map(m, batch.TAtom, t0d, batch.windows[t])
-        lnb = oracle.lnBtSG(om.F_mn, om.maxF)
-        F = b.F_d[int(b.F_off[t]):int(b.F_off[t + 1])].cpu().numpy().reshape(om.F_mn.shape)
-        mask = np.abs(om.cond / 1e4 - 1) < 1e-3
-        near += int(mask.sum())
-        rel = np.abs(F - om.F_mn) / np.maximum(np.abs(om.F_mn), 1e-30)
-        worst = max(worst, float(np.where(mask, 0, rel).max()))
-        worst_lnb = max(worst_lnb, abs(float(st[t]["lnBtSG"]) - lnb))
-        argmax_exact += int((int(st[t]["m_ML"]), int(st[t]["n_ML"])) == (om.m_max, om.n_max))
-    return {
-        "templates": len(idx),
+    worst, worst_lnb, argmax_exact, near, checked = 0.0, 0.0, 0, 0, 0
+    nunits = 1 if units.n is None else min(n, units.n)
+    tmpl_seen = set()
+    for i in range(nunits):
+        merged, t0d, w, TA, cells, (t, m) = units(i)
+        om = oracle.compute_map(merged, TA, t0d, w)
+        F = b.F_d[int(b.F_off[t]):int(b.F_off[t + 1])].cpu().numpy()
+        nt0, ntau = int(st[t]["N_t0"]), int(st[t]["N_tau"])
+        F = F.reshape(nt0, ntau)
+        if m is not None:
+            F = F[m:m + 1]
+        e, k = _err_figures(F, om.F_mn, om.cond)
+        worst = max(worst, e)
+        near += k
+        checked += F.size
+        if t in tmpl_seen:
+            continue
+        tmpl_seen.add(t)
+        if m is None:
+            lnb = oracle.lnBtSG(om.F_mn, om.maxF)
+            worst_lnb = max(worst_lnb, abs(float(st[t]["lnBtSG"]) - lnb))
+            argmax_exact += int((int(st[t]["m_ML"]), int(st[t]["n_ML"])) == (om.m_max, om.n_max))
+        else:
+            Fm = b.F_d[int(b.F_off[t]):int(b.F_off[t + 1])].cpu().numpy().astype(np.float64)
+            j = int(np.argmax(Fm))
+            argmax_exact += int((int(st[t]["m_ML"]), int(st[t]["n_ML"])) == divmod(j, ntau))
+            mx = Fm[j]
+            lnb = np.log(70.0 / Fm.size) + mx + np.log(np.exp(Fm - mx).sum())
+            worst_lnb = max(worst_lnb, abs(float(st[t]["lnBtSG"]) - lnb))
+    out = {
         "max_rel_err_2F": worst,
-        "argmax_exact": f"{argmax_exact}/{len(idx)}",
-        "max_abs_err_lnBtSG": worst_lnb,
+        "cells_checked": checked,
         "cells_near_cond_threshold": near,
+        "max_abs_err_lnBtSG": worst_lnb,
+    }
+    if units.rows:
+        out["rows"] = nunits
+        out["argmax_consistent"] = f"{argmax_exact}/{len(tmpl_seen)}"
+        out["lnBtSG_vs"] = "float64 logsumexp of the GPU map"
+    else:
+        out["templates"] = len(tmpl_seen)
+        out["argmax_exact"] = f"{argmax_exact}/{len(tmpl_seen)}"
+    return out
+
+
+def refine_cpu_baseline(prob, merged, cols, TA, budget_s, rng):
+    """The refinement's likelihood evaluation (2F + all derivatives at a
+    candidate) in the FP64 C oracle (-O3 -march=native, built here), OpenMP
+    over candidates: single core and all cores, on seeded-random candidates
+    of the workload at their start points, repeated to about budget_s."""
+    import oracle
+
+    flags = oracle.select_native()
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp:
+        cores = min(cores, int(omp))
+    cols_h = cols.cpu().numpy()
+    ct = np.asarray(prob.cand_tmpl)
+    lo, hi = np.asarray(prob.k_lo), np.asarray(prob.k_hi)
+    t0 = prob.t0_init.cpu().numpy()
+    tau = prob.tau_init.cpu().numpy()
+
+    def run(idx, threads, budget):
+        oracle.set_threads(threads)
+        args = (cols_h, merged.m_off, merged.t_data, TA, ct[idx], lo[idx], hi[idx], t0[idx],
+                tau[idx])
+        done, c0 = 0, time.perf_counter()
+        while True:
+            oracle.refine_eval_batch(*args)
+            done += idx.size
+            dt = time.perf_counter() - c0
+            if dt >= budget:
+                return done, dt
+
+    perm = rng.permutation(prob.n)
+    one = perm[:4096]
+    d1, t1 = run(one, 1, 0.3 * budget_s)
+    dn, tn = run(perm, cores, 0.7 * budget_s)
+    bins_all = int(np.asarray(prob.support_lengths).sum())
+    return {
+        "value": dn / tn,
+        "unit": "candidate-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "single_core": d1 / t1,
+        "cpu_model": oracle.cpu_model(),
+        "build": flags,
+        "sample": f"all {prob.n} candidates ({bins_all} support bins) at their start points, "
+                  f"{dn // prob.n} passes in {tn:.1f} s, FP64 C oracle evaluation (2F + all "
+                  f"derivatives, direct exp per bin), OpenMP over candidates; single core: "
+                  f"{one.size} seeded-random candidates, {d1 // one.size} passes in {t1:.1f} s",
     }
 
 
@@ -334,26 +511,16 @@ def run_refine(args):
                                   int(prob.k_hi[i]), float(t0_h[i]), float(lo_h[i]),
                                   float(hi_h[i]), n=401)
         scan_gap = max(scan_gap, (2 * Fmax - twoF_h[i]) / abs(2 * Fmax))
+    # every candidate at its ascent result against the C oracle (FP64)
+    import oracle
+
+    Fo, _, dtau_o, _ = oracle.refine_eval_batch(cols.cpu().numpy(), merged.m_off, merged.t_data,
+                                                TA, prob.cand_tmpl, prob.k_lo, prob.k_hi,
+                                                t0_h, tau_h)
+    worst_all = float(np.max(np.abs(twoF_h - 2 * Fo) / np.abs(2 * Fo)))
     cpu = None
     if not args.no_cpu_baseline:
-        # the FP64 numpy reference, one core: evaluations (2F + derivatives) at
-        # the candidates' start points, repeated until about --cpu-seconds pass
-        sample = rng.permutation(prob.n)
-        tau0_h = prob.tau_init.cpu().numpy()
-        done, bins = 0, 0
-        c0 = time.perf_counter()
-        while time.perf_counter() - c0 < args.cpu_seconds:
-            i = int(sample[done % prob.n])
-            t = int(prob.cand_tmpl[i])
-            sub = cols_h[:, merged.m_off[t]:merged.m_off[t + 1]]
-            rr.evaluate(sub, float(merged.t_data[t]), TA, int(prob.k_lo[i]), int(prob.k_hi[i]),
-                        float(t0_h[i]), float(tau0_h[i]))
-            done += 1
-            bins += int(L[i])
-        dt = time.perf_counter() - c0
-        cpu = {"value": done / dt, "unit": "candidate-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{done} random candidates ({bins} support bins, {dt:.1f} s), FP64 "
-                         "numpy reference evaluate() (2F + all derivatives), single core"}
+        cpu = refine_cpu_baseline(prob, merged, cols, TA, args.cpu_seconds, rng)
         log(f"cpu: {cpu}")
     flops = 42.0 * float(L.sum()) * (n_steps + 1)
     achieved = flops / (ms / 1e3) / 1e12
@@ -398,6 +565,7 @@ def run_refine(args):
             "candidates": int(idx.size),
             "max_rel_err_2F": worst,
             "max_rel_gap_to_scanned_max": scan_gap,
+            "all_candidates_max_rel_err_2F_vs_C_oracle": worst_all,
         },
         "cpu_baseline": cpu,
     }
@@ -492,8 +660,7 @@ def main():
 
         for item in args.rows:
             mode, rows = (int(x) for x in item.split(":"))
-            autotune.set_rows(mode, rows)
-        b.replan()
+            autotune.set_rows(b, mode, rows)
         tuned = {item.split(":")[0]: int(item.split(":")[1]) for item in args.rows}
         log(f"rows per item by mode (fixed): {tuned}")
     elif not args.no_autotune and not args.profile_mode:
@@ -544,7 +711,9 @@ def main():
     atoms_binned = int(b.stats()["numAtoms"].sum())
     copy_gbps = measured_copy_gbps(b.device)
     log(f"1 GiB device copy: {copy_gbps:.0f} GB/s")
-    parity = parity_sample(b, batch, args.check_templates) if batch is not None else None
+    units = Units(args, work, batch)
+    n_check = max(args.check_templates, 32) if units.rows else args.check_templates
+    parity = parity_sample(b, units, n_check)
     if parity:
         log(f"parity: {parity}")
 
@@ -566,9 +735,9 @@ def main():
         log(f"plain: {plain}")
 
     cpu = None
-    if not args.no_cpu_baseline and batch is not None:
+    if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        cpu = cpu_baseline(batch, args.cpu_seconds)
+        cpu = cpu_baseline(units, args.cpu_seconds)
         log(f"cpu: {cpu}")
 
     value = cells / (ms_per_step / 1e3)

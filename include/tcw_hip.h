@@ -146,6 +146,15 @@ typedef struct tcw_plan tcw_plan;
 int tcw_batch_plan(tcw_handle *h, const tcw_batch_desc *desc, uint32_t flags,
                    tcw_plan **out, int64_t *F_off);
 
+/* tcw_batch_plan with the rows (t0) per work item chosen per kernel mode:
+ * rows_by_mode (host [4], may be NULL = all 0) for modes 0 (rect, regular
+ * grid), 1 (rect, generic), 2 (exp, direct) and 3 (exp on the atom grid);
+ * each 0 (the library's choice for the batch) or a multiple of 16 in
+ * [16, 512] ([16, 128] for mode 3).  The setting belongs to this plan only.
+ * What the autotuner varies; no counterpart in the reference. */
+int tcw_batch_plan_rows(tcw_handle *h, const tcw_batch_desc *desc, uint32_t flags,
+                        const int32_t *rows_by_mode, tcw_plan **out, int64_t *F_off);
+
 /* Device bytes of scratch the plan holds (prefix sums, partials, tables). */
 int64_t tcw_plan_scratch_bytes(const tcw_plan *p);
 
@@ -173,11 +182,6 @@ int tcw_batch_run_timed(tcw_handle *h, tcw_plan *p, const uint32_t *ts,
                         tcw_stats *stats_dev, double *ms_out, void *stream);
 
 int tcw_plan_destroy(tcw_plan *p);
-
-/* Tuning hook used by the autotuner: rows of (t0) per work item for kernel
- * mode 0 (rect, regular grid), 1 (rect, generic) or 2 (exp); a multiple of
- * 16 in [16, 512], or 0 for the built-in default.  Affects later plans. */
-int tcw_set_rows_per_item(int mode, int rows);
 
 /* ---------------------------------------------------------------------------
  * Binned detector merge for many templates (lalpulsar

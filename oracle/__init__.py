@@ -57,20 +57,58 @@ class _Result(ctypes.Structure):
     ]
 
 
-def build():
-    """Compile tcw_oracle.c with the committed Makefile."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def build(native=False):
+    """Compile tcw_oracle.c with the committed Makefile (``native``: the
+    -march=native variant, for timing on the machine that builds it)."""
+    subprocess.run(["make", "-s", "-C", _HERE] + (["native"] if native else []), check=True)
 
 
 _lib = None
+_variant = "generic"
+
+
+def select_native():
+    """Build (on this machine) and use the -O3 -march=native library from now
+    on; returns its compiler flags for the record."""
+    global _lib, _LIB_PATH, _variant
+    build(native=True)
+    _LIB_PATH = os.path.join(_HERE, "build", "native", "libtcw_oracle.so")
+    _lib = None
+    _variant = "native"
+    _load()
+    return "gcc -O3 -march=native -ffp-contract=off -fopenmp"
+
+
+def set_threads(n):
+    """OpenMP threads of the batch functions; returns the previous maximum."""
+    return int(_load().orc_set_threads(int(n)))
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _load():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
-            build()
+            build(native=_variant == "native")
         lib = ctypes.CDLL(_LIB_PATH)
+        lib.orc_set_threads.restype = ctypes.c_int
+        lib.orc_set_threads.argtypes = [ctypes.c_int]
+        vp = ctypes.c_void_p
+        lib.orc_refine_eval_batch.restype = ctypes.c_int
+        lib.orc_refine_eval_batch.argtypes = [ctypes.c_int64, vp, ctypes.c_int64, vp, vp,
+                                              ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp,
+                                              vp, vp]
         P = ctypes.POINTER
         lib.orc_merge_size.restype = ctypes.c_int64
         lib.orc_merge_size.argtypes = [ctypes.c_int, P(ctypes.c_int64),
@@ -267,3 +305,30 @@ def compute_map_batch(merged_list, TAtoms, t0_datas, windows, with_lnBtSG=False)
         raise ValueError("oracle batch failed")
     Fs = [F[F_off[t]:F_off[t + 1]].reshape(shapes[t]) for t in range(T)]
     return Fs, maxF, m_max, n_max, lnB
+
+
+def refine_eval_batch(cols, m_off, t_data, TA, cand_tmpl, k_lo, k_hi, t0, tau):
+    """C restatement of refine_ref.batch_evaluate (FP64, direct exp per bin),
+    OpenMP over candidates.  cols float32 [7, n_bins] (merged layout, template
+    t at m_off[t]..).  Returns F [n], dt0 [n], dtau [n], g [n, 7]."""
+    cols = np.ascontiguousarray(cols, dtype=np.float32)
+    m_off = np.ascontiguousarray(m_off, dtype=np.int64)
+    t_data = np.ascontiguousarray(t_data, dtype=np.uint32)
+    ct = np.ascontiguousarray(cand_tmpl, dtype=np.int64)
+    lo = np.ascontiguousarray(k_lo, dtype=np.int64)
+    hi = np.ascontiguousarray(k_hi, dtype=np.int64)
+    t0 = np.ascontiguousarray(t0, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    n = ct.size
+    F = np.empty(n)
+    dt0 = np.empty(n)
+    dtau = np.empty(n)
+    g = np.empty((n, 7))
+    rc = _load().orc_refine_eval_batch(n, cols.ctypes.data, cols.shape[1], m_off.ctypes.data,
+                                       t_data.ctypes.data, int(TA), ct.ctypes.data,
+                                       lo.ctypes.data, hi.ctypes.data, t0.ctypes.data,
+                                       tau.ctypes.data, F.ctypes.data, dt0.ctypes.data,
+                                       dtau.ctypes.data, g.ctypes.data)
+    if rc:
+        raise ValueError("oracle refine batch failed")
+    return F, dt0, dtau, g

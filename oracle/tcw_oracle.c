@@ -321,3 +321,87 @@ int orc_map_batch(int64_t T, const float *atoms, const int64_t *atoms_off,
     }
     return err ? -1 : 0;
 }
+
+/* Threads for the OpenMP loops above (CPU baseline: single core vs all
+ * cores); returns the previous maximum. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+/* Exponential-window candidates on a fixed support (the refinement's
+ * likelihood call; the same window sums as the map, exp.cu:82-129, at a
+ * continuous (t0, tau)): per candidate i of template cand_tmpl[i], bins
+ * k_lo[i]..k_hi[i] of that template's merged atoms (cols is [7][col_stride],
+ * template t at columns m_off[t]..), bin k at t_data[t] + k*TAtom:
+ *   S_c = sum_k w^p_c x_ck,  U_c = sum_k w^p_c (t_k - t0) x_ck,
+ *   w = exp(-(t_k - t0)/tau), p = (2,2,2,1,1,1,1),
+ * F with the cond < 1e4 guard (F = 2 and zero derivatives otherwise), and
+ *   dF/dS_c (7), dF/dt0 = sum p S g / tau, dF/dtau = sum p U g / tau^2.
+ * A C restatement of oracle/refine_ref.py's evaluate() (FP64, direct exp per
+ * bin), OpenMP over candidates. */
+int orc_refine_eval_batch(int64_t n, const float *cols, int64_t col_stride,
+                          const int64_t *m_off, const uint32_t *t_data, uint32_t TAtom,
+                          const int64_t *cand_tmpl, const int64_t *k_lo,
+                          const int64_t *k_hi, const double *t0, const double *tau,
+                          double *F_out, double *dt0_out, double *dtau_out,
+                          double *g_out /* [n][7] */)
+{
+    static const int P[ORC_NCOL] = {2, 2, 2, 1, 1, 1, 1};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t t = cand_tmpl[i];
+        const float *x = cols + m_off[t];
+        double S[ORC_NCOL] = {0}, U[ORC_NCOL] = {0};
+        for (int64_t k = k_lo[i]; k <= k_hi[i]; k++) {
+            const double u = ((double)t_data[t] + (double)k * (double)TAtom) - t0[i];
+            const double w = exp(-u / tau[i]);
+            for (int c = 0; c < ORC_NCOL; c++) {
+                const double wp = P[c] == 2 ? w * w : w;
+                const double v = wp * (double)x[(int64_t)c * col_stride + k];
+                S[c] += v;
+                U[c] += v * u;
+            }
+        }
+        const double A = S[0], B = S[1], C = S[2];
+        const double far = S[3], fai = S[4], fbr = S[5], fbi = S[6];
+        double cond;
+        const double F = orc_F(A, B, C, far, fai, fbr, fbi, &cond);
+        double g[ORC_NCOL] = {0};
+        double d0 = 0.0, dt = 0.0;
+        if (cond < 1e4) {
+            const double D = A * B - C * C;
+            const double p = far * far + fai * fai;
+            const double q = fbr * fbr + fbi * fbi;
+            const double r = far * fbr + fai * fbi;
+            g[0] = (q - F * B) / D;
+            g[1] = (p - F * A) / D;
+            g[2] = 2.0 * (F * C - r) / D;
+            g[3] = 2.0 * (B * far - C * fbr) / D;
+            g[4] = 2.0 * (B * fai - C * fbi) / D;
+            g[5] = 2.0 * (A * fbr - C * far) / D;
+            g[6] = 2.0 * (A * fbi - C * fai) / D;
+            for (int c = 0; c < ORC_NCOL; c++) {
+                d0 += P[c] * S[c] * g[c];
+                dt += P[c] * U[c] * g[c];
+            }
+            d0 /= tau[i];
+            dt /= tau[i] * tau[i];
+        }
+        F_out[i] = F;
+        dt0_out[i] = d0;
+        dtau_out[i] = dt;
+        for (int c = 0; c < ORC_NCOL; c++) g_out[i * ORC_NCOL + c] = g[c];
+    }
+    return 0;
+}

@@ -40,12 +40,12 @@ EXPORTED_SYMBOLS = (
     "tcw_destroy",
     "tcw_map",
     "tcw_batch_plan",
+    "tcw_batch_plan_rows",
     "tcw_plan_scratch_bytes",
     "tcw_plan_items",
     "tcw_batch_run",
     "tcw_batch_run_timed",
     "tcw_plan_destroy",
-    "tcw_set_rows_per_item",
     "tcw_merge_batch",
     "tcw_gather_bins",
     "tcw_refine_plan_create",
@@ -173,6 +173,8 @@ def load():
     lib.tcw_map.argtypes = [vp, P(Atoms), P(Window), vp, P(Stats), ctypes.c_uint32, vp]
     lib.tcw_batch_plan.restype = ctypes.c_int
     lib.tcw_batch_plan.argtypes = [vp, P(BatchDesc), ctypes.c_uint32, P(vp), vp]
+    lib.tcw_batch_plan_rows.restype = ctypes.c_int
+    lib.tcw_batch_plan_rows.argtypes = [vp, P(BatchDesc), ctypes.c_uint32, vp, P(vp), vp]
     lib.tcw_plan_scratch_bytes.restype = ctypes.c_int64
     lib.tcw_plan_scratch_bytes.argtypes = [vp]
     lib.tcw_plan_items.restype = ctypes.c_int
@@ -183,8 +185,6 @@ def load():
     lib.tcw_batch_run_timed.argtypes = [vp, vp, vp, vp, ctypes.c_int64, vp, vp, vp, vp]
     lib.tcw_plan_destroy.restype = ctypes.c_int
     lib.tcw_plan_destroy.argtypes = [vp]
-    lib.tcw_set_rows_per_item.restype = ctypes.c_int
-    lib.tcw_set_rows_per_item.argtypes = [ctypes.c_int, ctypes.c_int]
     i64 = ctypes.c_int64
     lib.tcw_merge_batch.restype = ctypes.c_int
     lib.tcw_merge_batch.argtypes = [vp, P(BatchDesc), vp, vp, i64, vp, i64, vp, vp, vp, vp]

@@ -6,9 +6,11 @@ rows amortise the item's set-up (diagonal staging for the rect kernel, the
 chunk walk for the exp recurrence) and the per-item reductions; fewer rows give
 more items to spread over the 256 CUs and a smaller LDS footprint.  The best
 value depends on the batch's shapes, so it is measured: each candidate value
-is planned (``tcw_set_rows_per_item`` + ``BatchedTransientMaps.replan``) and
-the step's kernels timed with HIP events (``run_timed``); the fastest is kept
-and cached per (device, mode, shape class).
+is planned (``BatchedTransientMaps.set_rows`` + ``replan``, which pass it to
+``tcw_batch_plan_rows``) and the step's kernels timed with HIP events
+(``run_timed``); the fastest is kept and cached per (device, mode, shape
+class).  The setting belongs to the tuned batch's plan: other batches and
+single-map calls keep the library's own choice.
 
 The reference has no counterpart (its kernels use fixed 1-D launches,
 pyfstat/tcw_fstat_map_funcs.py:862-870, 944-952).
@@ -28,9 +30,6 @@ CANDIDATES = {
     _hiplib.MODE_EXP_DIRECT: (16, 32, 64),
     _hiplib.MODE_EXP_FAST: (32, 64, 96, 128),
 }
-
-_applied = {}
-
 
 def cache_path():
     env = os.environ.get("PYFSTAT_AMD_TUNE_CACHE")
@@ -74,17 +73,25 @@ def shape_class(b, mode, device_name=""):
             f"|rows{_bucket(np.median(shapes[:, 0]))}|cols{_bucket(np.median(shapes[:, 1]))}")
 
 
-def set_rows(mode, rows):
-    lib = _hiplib.load()
-    _hiplib.check(lib.tcw_set_rows_per_item(int(mode), int(rows)), "tcw_set_rows_per_item")
-    _applied[int(mode)] = int(rows)
+def set_rows(b, mode, rows):
+    """Plan batch ``b`` with ``rows`` per work item for ``mode`` (0: the
+    library's choice); raises ValueError for a value the library rejects,
+    and then restores the previous value and plan."""
+    prev = int(b.rows_by_mode[int(mode)])
+    b.set_rows(mode, rows)
+    try:
+        b.replan()
+    except ValueError:
+        b.set_rows(mode, prev)
+        b.replan()
+        raise
 
 
-def reset():
-    """Restore the built-in defaults for every mode."""
+def reset(b):
+    """Back to the library's choice for every mode of batch ``b``."""
     for mode in range(_hiplib.MODE_COUNT):
-        set_rows(mode, 0)
-    _applied.clear()
+        b.set_rows(mode, 0)
+    b.replan()
 
 
 def _time_step(b, reps):
@@ -97,7 +104,7 @@ def _time_step(b, reps):
 
 def tune(b, reps=3, candidates=None, use_cache=True, device_name=None, log=None):
     """Pick rows per item for every kernel mode the batch uses; leaves the
-    winners set (and the batch re-planned with them).  Returns
+    winners set on ``b`` (re-planned with them).  Returns
     {mode: {"rows": r, "step_ms": t, "tried": {r: t}, "cached": bool}}."""
     from .tcw_fstat_map_funcs import _hip_device_names, _HipState
 
@@ -117,7 +124,7 @@ def tune(b, reps=3, candidates=None, use_cache=True, device_name=None, log=None)
         key = shape_class(b, mode, device_name)
         if use_cache and key in cache:
             try:
-                set_rows(mode, cache[key]["rows"])
+                set_rows(b, mode, cache[key]["rows"])
             except (ValueError, KeyError, TypeError):
                 # a value this library no longer accepts (its limits changed
                 # under the same version string): tune again
@@ -128,13 +135,12 @@ def tune(b, reps=3, candidates=None, use_cache=True, device_name=None, log=None)
                 continue
         tried = {}
         for rows in cands[mode]:
-            set_rows(mode, rows)
-            b.replan()
+            set_rows(b, mode, rows)
             tried[rows] = _time_step(b, reps)
             if log:
                 log(f"autotune mode {mode}: rows {rows} -> step {tried[rows]:.4f} ms")
         best = min(tried, key=tried.get)
-        set_rows(mode, best)
+        b.set_rows(mode, best)
         changed = True
         result[mode] = {"rows": int(best), "step_ms": float(tried[best]),
                         "tried": {str(k): float(v) for k, v in tried.items()}, "cached": False}

@@ -38,7 +38,7 @@ class BatchedTransientMaps:
     """
 
     def __init__(self, ts, cols, seg_off, seg_begin, windows, TAtom, BtSG=False,
-                 lal_strict=False, device=None, force_generic=False):
+                 lal_strict=False, device=None, force_generic=False, rows=None):
         from .tcw_fstat_map_funcs import _HipState, _ensure_handle
 
         torch = _torch()
@@ -82,6 +82,11 @@ class BatchedTransientMaps:
         desc.windows = ctypes.addressof(self.win_arr)
         self._desc = desc
         self.F_off = np.zeros(self.T + 1, dtype=np.int64)
+        # rows (t0) per work item per kernel mode, 0 = the library's choice;
+        # a setting of this batch's plan only
+        self.rows_by_mode = np.zeros(_hiplib.MODE_COUNT, dtype=np.int32)
+        for mode, r in (rows or {}).items():
+            self.rows_by_mode[int(mode)] = int(r)
         self.plan = None
         self._make_plan()
         self.ts_d = torch.from_numpy(ts.view(np.int32)).to(self.device)
@@ -93,14 +98,21 @@ class BatchedTransientMaps:
 
     def _make_plan(self):
         plan = ctypes.c_void_p()
-        _hiplib.check(self.lib.tcw_batch_plan(self.handle, ctypes.byref(self._desc), self.flags,
-                                              ctypes.byref(plan), self.F_off.ctypes.data),
-                      "tcw_batch_plan")
+        _hiplib.check(self.lib.tcw_batch_plan_rows(self.handle, ctypes.byref(self._desc),
+                                                   self.flags, self.rows_by_mode.ctypes.data,
+                                                   ctypes.byref(plan), self.F_off.ctypes.data),
+                      "tcw_batch_plan_rows")
         self.plan = plan
+
+    def set_rows(self, mode, rows):
+        """Rows per work item for one kernel mode (0: the library's choice);
+        takes effect at the next ``replan``."""
+        self.rows_by_mode[int(mode)] = int(rows)
 
     def replan(self):
         """Rebuild the plan (work items, scratch, graph) with the current
-        rows-per-item settings; atoms and outputs stay where they are."""
+        rows-per-item settings; atoms and outputs stay where they are.  A
+        setting the library rejects raises ValueError and keeps no plan."""
         self.close()
         self._make_plan()
 

@@ -1765,7 +1765,6 @@ using tcwc::DevBuf;
 using tcwc::fail;
 using tcwc::set_device;
 
-int g_rb_override[MODE_COUNT] = {0, 0, 0, 0};
 
 }  // namespace
 
@@ -1795,6 +1794,8 @@ struct tcw_plan {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     const void *g_ts = nullptr, *g_cols = nullptr, *g_F = nullptr, *g_stats = nullptr;
     int64_t g_cstride = -1;
+    // rows per work item requested for each kernel mode (0: built-in choice)
+    int32_t rows_req[MODE_COUNT] = {0, 0, 0, 0};
     void reset() {
         tmpl.clear();
         segs.clear();
@@ -1945,8 +1946,9 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         }
     }
     td.mode = mode;
-    int rb = g_rb_override[mode] > 0 ? g_rb_override[mode] : RB_DEFAULT[mode];
-    if (mode == MODE_RECT_FAST && g_rb_override[mode] <= 0) {
+    const int req = p.rows_req[mode];  // the plan's tuned rows, 0 = built-in choice
+    int rb = req > 0 ? req : RB_DEFAULT[mode];
+    if (mode == MODE_RECT_FAST && req <= 0) {
         // few or small maps (the single-map call): fewer rows per item, so
         // that the batch's items cover the CUs (about SMALL_BATCH_ITEMS of them
         // if the templates are alike)
@@ -1955,7 +1957,7 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         rb = want >= RB_DEFAULT[mode] ? RB_DEFAULT[mode]
                                       : std::max(16, ((int)std::ceil(want) + 15) / 16 * 16);
     }
-    if (mode == MODE_EXP_FAST && g_rb_override[mode] <= 0 &&
+    if (mode == MODE_EXP_FAST && req <= 0 &&
         p.T * ((N_tau + TILE_COLS - 1) / TILE_COLS) < DIRECT_INIT_MIN_BLOCKS)
         rb = EXP_SMALL_BATCH_ROWS;  // few column blocks: shorter chunks, more of them
     td.rb_rows = rb;
@@ -2012,6 +2014,24 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
     return TCW_OK;
 }
 
+// Every device buffer of a plan with the bytes the plan needs in it.
+template <typename F>
+void plan_buffers(tcw_plan &p, F f) {
+    f(p.d_tmpl, sizeof(TmplDev) * std::max<size_t>(p.tmpl.size(), 1));
+    f(p.d_segs, sizeof(SegDev) * std::max<size_t>(p.segs.size(), 1));
+    f(p.d_items, sizeof(ItemDev) * std::max<size_t>(p.items.size(), 1));
+    f(p.d_prefix, sizeof(double) * std::max<int64_t>(p.prefix_doubles, 1));
+    f(p.d_merged, sizeof(double) * std::max<int64_t>(p.merged_doubles, 1));
+    f(p.d_ipart, sizeof(ItemPart) * std::max<size_t>(p.items.size(), 1));
+    f(p.d_rowpart, sizeof(float) * std::max<int64_t>(p.rowpart_n, 1));
+    f(p.d_colpart, sizeof(float) * std::max<int64_t>(p.colpart_n, 1));
+    f(p.d_flags, sizeof(int) * std::max<int64_t>(p.T, 1));
+    f(p.d_seg_items, sizeof(ItemDev) * std::max<size_t>(p.seg_items.size(), 1));
+    f(p.d_col_items, sizeof(ItemDev) * std::max<size_t>(p.col_items.size(), 1));
+    f(p.d_etab, sizeof(double) * std::max<int64_t>(p.etab_doubles, 1));
+    f(p.d_exp_tmpls, sizeof(int32_t) * std::max<size_t>(p.exp_tmpls.size(), 1));
+}
+
 // After all templates: lay out items mode by mode, fix item_begin (the items
 // of one template stay contiguous, rb-major), size scratch, upload descriptors
 // (asynchronously on `st` when given, else synchronously).
@@ -2028,23 +2048,11 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
         }
     }
     p.mode_begin[MODE_COUNT] = (int64_t)p.items.size();
-    int rc;
-    if ((rc = p.d_tmpl.ensure(sizeof(TmplDev) * p.tmpl.size()))) return rc;
-    if ((rc = p.d_segs.ensure(sizeof(SegDev) * std::max<size_t>(p.segs.size(), 1)))) return rc;
-    if ((rc = p.d_items.ensure(sizeof(ItemDev) * std::max<size_t>(p.items.size(), 1)))) return rc;
-    if ((rc = p.d_prefix.ensure(sizeof(double) * std::max<int64_t>(p.prefix_doubles, 1)))) return rc;
-    if ((rc = p.d_merged.ensure(sizeof(double) * std::max<int64_t>(p.merged_doubles, 1)))) return rc;
-    if ((rc = p.d_ipart.ensure(sizeof(ItemPart) * std::max<size_t>(p.items.size(), 1)))) return rc;
-    if ((rc = p.d_rowpart.ensure(sizeof(float) * std::max<int64_t>(p.rowpart_n, 1)))) return rc;
-    if ((rc = p.d_colpart.ensure(sizeof(float) * std::max<int64_t>(p.colpart_n, 1)))) return rc;
-    if ((rc = p.d_flags.ensure(sizeof(int) * std::max<int64_t>(p.T, 1)))) return rc;
-    if ((rc = p.d_seg_items.ensure(sizeof(ItemDev) * std::max<size_t>(p.seg_items.size(), 1))))
-        return rc;
-    if ((rc = p.d_col_items.ensure(sizeof(ItemDev) * std::max<size_t>(p.col_items.size(), 1))))
-        return rc;
-    if ((rc = p.d_etab.ensure(sizeof(double) * std::max<int64_t>(p.etab_doubles, 1)))) return rc;
-    if ((rc = p.d_exp_tmpls.ensure(sizeof(int32_t) * std::max<size_t>(p.exp_tmpls.size(), 1))))
-        return rc;
+    int rc = TCW_OK;
+    plan_buffers(p, [&](DevBuf &b, size_t bytes) {
+        if (!rc) rc = b.ensure(bytes);
+    });
+    if (rc) return rc;
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
         return async ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
@@ -2059,11 +2067,29 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     return TCW_OK;
 }
 
-// Plan `d` into `p` (buffers reused, grow-only).
+// Rows per work item a caller may request for kernel mode `mode`.
+int check_rows(int mode, int rows) {
+    const int maxr = mode == MODE_EXP_FAST ? MAX_RB_EXPF : MAX_RB;
+    if (rows != 0 && (rows < 16 || rows > maxr || rows % 16))
+        return fail(TCW_ERR_INVALID, "rows per item for mode " + std::to_string(mode) +
+                                         " must be 0 or a multiple of 16 in [16, " +
+                                         std::to_string(maxr) + "], not " + std::to_string(rows));
+    return TCW_OK;
+}
+
+// Plan `d` into `p` (buffers reused, grow-only).  rows_by_mode: host [4] or
+// null (built-in choices).
 int plan_build(tcw_plan &p, int device, const tcw_batch_desc *d, uint32_t flags,
-               int64_t *F_off, hipStream_t st, bool async) {
+               int64_t *F_off, hipStream_t st, bool async,
+               const int32_t *rows_by_mode = nullptr) {
     if (d->T <= 0) return fail(TCW_ERR_INVALID, "no templates");
     if (d->TAtom < 2) return fail(TCW_ERR_INVALID, "TAtom must be at least 2 seconds.");
+    for (int md = 0; md < MODE_COUNT; md++) {
+        const int r = rows_by_mode ? rows_by_mode[md] : 0;
+        int rc = check_rows(md, r);
+        if (rc) return rc;
+        p.rows_req[md] = r;
+    }
     p.reset();
     p.device = device;
     p.flags = flags;
@@ -2300,11 +2326,16 @@ int tcw_destroy(tcw_handle *h) {
 
 int tcw_batch_plan(tcw_handle *h, const tcw_batch_desc *d, uint32_t flags, tcw_plan **out,
                    int64_t *F_off) {
+    return tcw_batch_plan_rows(h, d, flags, nullptr, out, F_off);
+}
+
+int tcw_batch_plan_rows(tcw_handle *h, const tcw_batch_desc *d, uint32_t flags,
+                        const int32_t *rows_by_mode, tcw_plan **out, int64_t *F_off) {
     if (!h || !d || !out) return fail(TCW_ERR_INVALID, "tcw_batch_plan: null argument");
     int rc = set_device(h->device);
     if (rc) return rc;
     auto *p = new tcw_plan;
-    rc = plan_build(*p, h->device, d, flags, F_off, nullptr, false);
+    rc = plan_build(*p, h->device, d, flags, F_off, nullptr, false, rows_by_mode);
     if (rc) {
         delete p;
         return rc;
@@ -2410,16 +2441,20 @@ int tcw_plan_destroy(tcw_plan *p) {
     return TCW_OK;
 }
 
-int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win, float *F_mn_dev,
-            tcw_stats *stats, uint32_t flags, void *stream) {
-    if (!h || !atoms || !win || !F_mn_dev || !stats)
-        return fail(TCW_ERR_INVALID, "tcw_map: null argument");
+}  // extern "C"
+
+namespace {
+
+// One template as a batch of one (tcw_map): checks the
+// atoms and plans `p` on `st` (descriptor uploads queued on `st`).
+int plan_single(tcw_handle *h, tcw_plan &p, const tcw_atoms *atoms, const tcw_window *win,
+                uint32_t flags, hipStream_t st, const char *who) {
     if (atoms->numDet < 1 || atoms->numDet > TCW_MAX_DETECTORS)
         return fail(TCW_ERR_INVALID,
-                    "tcw_map: numDet must be 1.." + std::to_string(TCW_MAX_DETECTORS));
-    if (!atoms->ts || !atoms->cols) return fail(TCW_ERR_INVALID, "tcw_map: null atoms");
+                    std::string(who) + ": numDet must be 1.." + std::to_string(TCW_MAX_DETECTORS));
+    if (!atoms->ts || !atoms->cols) return fail(TCW_ERR_INVALID, std::string(who) + ": null atoms");
     if (atoms->t_max < atoms->t_min)
-        return fail(TCW_ERR_INVALID, "tcw_map: t_max < t_min");
+        return fail(TCW_ERR_INVALID, std::string(who) + ": t_max < t_min");
     int rc = set_device(h->device);
     if (rc) return rc;
     int64_t seg_begin[2] = {0, (int64_t)atoms->numDet};
@@ -2437,8 +2472,19 @@ int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win, float 
     d.seg_tfirst = tfirst;
     d.seg_tlast = tlast;
     d.windows = win;
+    return plan_build(p, h->device, &d, flags, nullptr, st, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win, float *F_mn_dev,
+            tcw_stats *stats, uint32_t flags, void *stream) {
+    if (!h || !atoms || !win || !F_mn_dev || !stats)
+        return fail(TCW_ERR_INVALID, "tcw_map: null argument");
     hipStream_t st = (hipStream_t)stream;
-    rc = plan_build(h->single, h->device, &d, flags, nullptr, st, true);
+    int rc = plan_single(h, h->single, atoms, win, flags, st, "tcw_map");
     if (rc) return rc;
     rc = launch_all(h->single, atoms->ts, atoms->cols, atoms->col_stride, F_mn_dev,
                     h->d_stats, st);
@@ -2451,17 +2497,6 @@ int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win, float 
         return fail(TCW_ERR_SINGLE_ATOM,
                     "encountered a single-atom Fstat-calculation. This is degenerate and cannot "
                     "be computed! t0 must stay away *at least* 2*TAtom from the end of the data.");
-    return TCW_OK;
-}
-
-// Testing/tuning hook: rows per work item for a kernel mode (0 restores the default).
-int tcw_set_rows_per_item(int mode, int rows) {
-    if (mode < 0 || mode >= MODE_COUNT) return fail(TCW_ERR_INVALID, "bad mode");
-    const int maxr = mode == MODE_EXP_FAST ? MAX_RB_EXPF : MAX_RB;
-    if (rows != 0 && (rows < 16 || rows > maxr || rows % 16))
-        return fail(TCW_ERR_INVALID, "rows per item must be a multiple of 16 in [16, " +
-                                         std::to_string(maxr) + "]");
-    g_rb_override[mode] = rows;
     return TCW_OK;
 }
 

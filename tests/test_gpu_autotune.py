@@ -36,14 +36,8 @@ def _oracle_maps(ts, cols, seg_off, seg_begin, windows):
     return out
 
 
-@pytest.fixture(autouse=True)
-def _restore_defaults(hip_ready):
-    yield
-    autotune.reset()
-
-
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
-def test_every_candidate_rows_value_matches_oracle(mode):
+def test_every_candidate_rows_value_matches_oracle(hip_ready, mode):
     from pyfstat_amd.batched import BatchedTransientMaps
 
     ts, cols, seg_off, seg_begin, windows = _work(mode)
@@ -54,8 +48,7 @@ def test_every_candidate_rows_value_matches_oracle(mode):
     assert b.items_per_mode()[mode] > 0
     for rows in autotune.CANDIDATES[mode]:
         print(f"mode {mode} rows {rows}", flush=True)
-        autotune.set_rows(mode, rows)
-        b.replan()
+        autotune.set_rows(b, mode, rows)
         b.run()
         maps = b.maps()
         for t, om in enumerate(oms):
@@ -64,6 +57,32 @@ def test_every_candidate_rows_value_matches_oracle(mode):
             check_argmax(maps[t].F_mn, int(m), int(n), om)
             assert abs(maps[t].lnBtSG - om.lnBtSG) <= LNB_ATOL + 2e-7 * abs(om.maxF)
     b.close()
+
+
+def test_rows_belong_to_one_plan(hip_ready):
+    """Rows set on one batch change neither another batch's plan nor the
+    single-map call; a value the library rejects keeps the previous plan."""
+    from pyfstat_amd import tcw_fstat_map_funcs as tcw
+    from pyfstat_amd.batched import BatchedTransientMaps
+
+    ts, cols, seg_off, seg_begin, windows = _work(_hiplib.MODE_RECT_FAST)
+    fresh = BatchedTransientMaps(ts, cols, seg_off, seg_begin, windows, synth.TSFT, BtSG=True)
+    items0 = fresh.items_per_mode().copy()
+    tuned = BatchedTransientMaps(ts, cols, seg_off, seg_begin, windows, synth.TSFT, BtSG=True)
+    autotune.set_rows(tuned, _hiplib.MODE_RECT_FAST, 64)
+    assert not np.array_equal(tuned.items_per_mode(), items0)
+    with pytest.raises(ValueError, match="multiple of 16"):
+        autotune.set_rows(tuned, _hiplib.MODE_RECT_FAST, 24)
+    assert tuned.rows_by_mode[_hiplib.MODE_RECT_FAST] == 64
+    other = BatchedTransientMaps(ts, cols, seg_off, seg_begin, windows, synth.TSFT, BtSG=True)
+    np.testing.assert_array_equal(other.items_per_mode(), items0)
+    # the single-map call on the first template still matches the oracle
+    multi = synth.multi_from_detectors([(ts[seg_off[0]:seg_off[1]],
+                                         cols[:, seg_off[0]:seg_off[1]])])
+    fm = tcw.hip_compute_transient_fstat_map(multi, windows[0], BtSG=True)
+    compare_maps(fm.F_mn, _oracle_maps(ts, cols, seg_off, seg_begin, windows[:1])[0])
+    for b in (fresh, tuned, other):
+        b.close()
 
 
 def test_tune_picks_applies_and_caches(tmp_path, monkeypatch):

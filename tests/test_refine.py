@@ -129,3 +129,36 @@ def test_scan_max_brackets_the_grid_maximum(small_exp):
     Fmax, tmax = rr.scan_max_tau(merged, float(t0d), TA, kl, kh, float(t0),
                                  float(max(tau - win.dtau, TA)), float(tau + win.dtau), n=201)
     assert Fmax >= om.F_mn[m, n] - 1e-9
+
+
+def test_c_refine_oracle_equals_numpy_reference(small_exp):
+    """oracle.refine_eval_batch (C, OpenMP; the refinement's CPU baseline) is
+    the numpy reference evaluate(), candidate by candidate, on two templates
+    (the second one's atoms scaled, so a template mix-up shows)."""
+    merged, t0d, win, om = small_exp
+    N = merged.shape[1]
+    cols = np.concatenate([merged, 3.0 * merged], axis=1).astype(np.float32)
+    m_off = np.array([0, N, 2 * N])
+    t_data = np.array([t0d, t0d + 5 * TA], dtype=np.uint32)
+    rng = np.random.default_rng(7)
+    ct, lo, hi, t0s, taus = [], [], [], [], []
+    for i in range(80):
+        t = i % 2
+        m, n = int(rng.integers(0, om.F_mn.shape[0])), int(rng.integers(0, om.F_mn.shape[1]))
+        t0 = int(t_data[t]) + m * win.dt0
+        tau = win.tau + n * win.dtau
+        sup = rr.support_bruteforce(t0, tau, int(t_data[t]), N, TA)
+        if sup is None:
+            continue
+        ct.append(t)
+        lo.append(sup[0])
+        hi.append(sup[1])
+        t0s.append(t0 + rng.uniform(-0.3, 0.3) * TA)
+        taus.append(tau * rng.uniform(0.8, 1.25))
+    F, dt0, dtau, g = oracle.refine_eval_batch(cols, m_off, t_data, TA, ct, lo, hi, t0s, taus)
+    Fr, dt0r, dtaur, gr = rr.batch_evaluate(cols.astype(np.float64), m_off, t_data, TA, ct, lo,
+                                            hi, np.array(t0s), np.array(taus))
+    np.testing.assert_allclose(F, Fr, rtol=1e-12)
+    np.testing.assert_allclose(dtau, dtaur, rtol=1e-9, atol=1e-12 * np.abs(Fr).max() / np.min(taus))
+    np.testing.assert_allclose(g, gr, rtol=1e-9, atol=1e-12 * np.abs(gr).max())
+    np.testing.assert_allclose(dt0, dt0r, atol=1e-9 * np.abs(Fr).max() / np.min(taus))
