@@ -118,6 +118,27 @@ int tcw_destroy(tcw_handle *h);
 int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win,
             float *F_mn_dev, tcw_stats *stats, uint32_t flags, void *stream);
 
+/* Bytes of the library's static device scratch used by tcw_map_noalloc (one
+ * copy per device, held from library load until the process ends). */
+#define TCW_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* tcw_map without any device or pinned-memory allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels as tcw_map, with every intermediate buffer carved from the static
+ * scratch of h's device (looked up once by tcw_create).
+ * stats_dev: DEVICE tcw_stats, written by the last kernel; the call does not
+ * wait for the device (read stats_dev after synchronising `stream`; under
+ * TCW_FLAG_LAL_STRICT check stats_dev->single_atom yourself).
+ * Fails with TCW_ERR_INVALID, before queueing anything, when the map needs
+ * more than TCW_NOALLOC_SCRATCH_BYTES (e.g. an exp map of 2000 x 2000 cells on
+ * 4000 atoms needs ~9 MB; a rect map of 2000 x 2000 cells ~0.6 MB).
+ * All calls on one device share the scratch: calls that overlap in time on
+ * that device (different streams or host threads) race and must be ordered
+ * by the caller; the descriptors are uploaded with hipMemcpyAsync on `stream`. */
+int tcw_map_noalloc(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win,
+                    float *F_mn_dev, tcw_stats *stats_dev, uint32_t flags, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Batched maps: many templates (ragged atom counts and windows) in one pass.
  * Replaces the per-template Python loop of TransientGridSearch.run

@@ -24,6 +24,9 @@ TCW_FLAG_BTSG = 1
 TCW_FLAG_LAL_STRICT = 2
 TCW_FLAG_GENERIC = 4
 
+# static device scratch of tcw_map_noalloc (TCW_NOALLOC_SCRATCH_BYTES)
+TCW_NOALLOC_SCRATCH_BYTES = 32 << 20
+
 MODE_RECT_FAST = 0
 MODE_RECT_GENERIC = 1
 MODE_EXP_DIRECT = 2
@@ -39,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "tcw_create",
     "tcw_destroy",
     "tcw_map",
+    "tcw_map_noalloc",
     "tcw_batch_plan",
     "tcw_batch_plan_rows",
     "tcw_plan_scratch_bytes",
@@ -171,6 +175,8 @@ def load():
     lib.tcw_destroy.argtypes = [vp]
     lib.tcw_map.restype = ctypes.c_int
     lib.tcw_map.argtypes = [vp, P(Atoms), P(Window), vp, P(Stats), ctypes.c_uint32, vp]
+    lib.tcw_map_noalloc.restype = ctypes.c_int
+    lib.tcw_map_noalloc.argtypes = [vp, P(Atoms), P(Window), vp, vp, ctypes.c_uint32, vp]
     lib.tcw_batch_plan.restype = ctypes.c_int
     lib.tcw_batch_plan.argtypes = [vp, P(BatchDesc), ctypes.c_uint32, P(vp), vp]
     lib.tcw_batch_plan_rows.restype = ctypes.c_int

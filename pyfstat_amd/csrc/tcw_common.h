@@ -27,12 +27,21 @@ inline int fail(int code, const std::string &msg) {
             return ::tcwc::fail(TCW_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Grow-only device buffer.
+// Grow-only device buffer, or (view = true) a window into memory the buffer
+// does not own, which it never frees or grows.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool view = false;
+    void set_view(void *q, size_t bytes) {
+        release();
+        p = q;
+        cap = bytes;
+        view = true;
+    }
     int ensure(size_t bytes) {
         if (bytes <= cap) return TCW_OK;
+        if (view) return fail(TCW_ERR_INVALID, "internal: a scratch view cannot grow");
         if (p) {
             (void)hipFree(p);
             p = nullptr;
@@ -47,9 +56,10 @@ struct DevBuf {
         return TCW_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !view) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        view = false;
     }
 };
 

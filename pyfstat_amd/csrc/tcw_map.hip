@@ -26,11 +26,20 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "tcw_common.h"
 #include "tcw_hip.h"
+
+// Device memory of tcw_map_noalloc: the single-map plan's buffers (descriptors,
+// prefix or atom tables, reduction partials) are carved out of this array, so
+// that call allocates nothing.  Each device holds its own copy from module
+// load until the process ends; host code never uses this name as an address
+// (it is a host-side stand-in there), only the pointer hipGetSymbolAddress
+// returns for the current device (noalloc_scratch_ptr below).
+static __device__ __align__(256) unsigned char scratch[TCW_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -1796,6 +1805,9 @@ struct tcw_plan {
     int64_t g_cstride = -1;
     // rows per work item requested for each kernel mode (0: built-in choice)
     int32_t rows_req[MODE_COUNT] = {0, 0, 0, 0};
+    // when set, device buffers are views into this memory (tcw_map_noalloc)
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
     void reset() {
         tmpl.clear();
         segs.clear();
@@ -1838,6 +1850,7 @@ struct tcw_plan {
 struct tcw_handle {
     int device = 0;
     tcw_plan single;
+    tcw_plan noalloc;  // tcw_map_noalloc: buffers are views into the static scratch
     tcw_stats *d_stats = nullptr;
     tcw_stats *h_stats = nullptr;  // pinned
     DevBuf d_mtab, d_mseg, d_msegs;  // tcw_merge_batch tables
@@ -1851,6 +1864,45 @@ struct tcw_handle {
 };
 
 int tcwc::handle_device(const tcw_handle *h) { return h->device; }
+
+namespace {
+
+// Per-device address of the static scratch, looked up once per device (with
+// that device current) by tcw_create; hipGetSymbolAddress's status is kept,
+// so tcw_map_noalloc reports a failed lookup instead of using a bad address.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+hipError_t g_scratch_err[NOALLOC_MAX_DEVICES] = {};
+bool g_scratch_done[NOALLOC_MAX_DEVICES] = {};
+
+void noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+    g_scratch_done[device] = true;
+}
+
+int noalloc_scratch_ptr(int device, char **out) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return fail(TCW_ERR_INVALID, "tcw_map_noalloc: device " + std::to_string(device) +
+                                         " beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device])
+        return fail(TCW_ERR_INVALID, "tcw_map_noalloc: no handle was created on device " +
+                                         std::to_string(device));
+    if (g_scratch_err[device] != hipSuccess || !g_scratch_ptr[device])
+        return fail(TCW_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                        hipGetErrorString(g_scratch_err[device]));
+    *out = (char *)g_scratch_ptr[device];
+    return TCW_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -2033,8 +2085,9 @@ void plan_buffers(tcw_plan &p, F f) {
 }
 
 // After all templates: lay out items mode by mode, fix item_begin (the items
-// of one template stay contiguous, rb-major), size scratch, upload descriptors
-// (asynchronously on `st` when given, else synchronously).
+// of one template stay contiguous, rb-major), size scratch (own buffers, or
+// views into p.arena when set), upload descriptors (asynchronously on `st`
+// when given, else synchronously).
 int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
                 hipStream_t st, bool async) {
     p.items.clear();
@@ -2049,10 +2102,24 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     }
     p.mode_begin[MODE_COUNT] = (int64_t)p.items.size();
     int rc = TCW_OK;
-    plan_buffers(p, [&](DevBuf &b, size_t bytes) {
-        if (!rc) rc = b.ensure(bytes);
-    });
-    if (rc) return rc;
+    if (p.arena) {
+        // carve every buffer out of the caller-provided arena (no allocation)
+        size_t off = 0;
+        plan_buffers(p, [&](DevBuf &b, size_t bytes) {
+            const size_t a = (bytes + 255) / 256 * 256;
+            if (off + a <= p.arena_bytes) b.set_view(p.arena + off, a);
+            off += a;
+        });
+        if (off > p.arena_bytes)
+            return fail(TCW_ERR_INVALID, "this map needs " + std::to_string(off) +
+                                             " bytes of scratch; the static scratch holds " +
+                                             std::to_string(p.arena_bytes) + " bytes");
+    } else {
+        plan_buffers(p, [&](DevBuf &b, size_t bytes) {
+            if (!rc) rc = b.ensure(bytes);
+        });
+        if (rc) return rc;
+    }
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (bytes == 0) return hipSuccess;
         return async ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
@@ -2309,6 +2376,7 @@ int tcw_create(int device, tcw_handle **out) {
     if (rc) return rc;
     auto *h = new tcw_handle;
     h->device = device;
+    noalloc_scratch_setup(device);  // with `device` current
     if (hipMalloc(&h->d_stats, sizeof(tcw_stats)) != hipSuccess ||
         hipHostMalloc(&h->h_stats, sizeof(tcw_stats), hipHostMallocDefault) != hipSuccess) {
         delete h;
@@ -2445,7 +2513,7 @@ int tcw_plan_destroy(tcw_plan *p) {
 
 namespace {
 
-// One template as a batch of one (tcw_map): checks the
+// One template as a batch of one (tcw_map and tcw_map_noalloc): checks the
 // atoms and plans `p` on `st` (descriptor uploads queued on `st`).
 int plan_single(tcw_handle *h, tcw_plan &p, const tcw_atoms *atoms, const tcw_window *win,
                 uint32_t flags, hipStream_t st, const char *who) {
@@ -2498,6 +2566,26 @@ int tcw_map(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win, float 
                     "encountered a single-atom Fstat-calculation. This is degenerate and cannot "
                     "be computed! t0 must stay away *at least* 2*TAtom from the end of the data.");
     return TCW_OK;
+}
+
+int tcw_map_noalloc(tcw_handle *h, const tcw_atoms *atoms, const tcw_window *win,
+                    float *F_mn_dev, tcw_stats *stats_dev, uint32_t flags, void *stream) {
+    if (!h || !atoms || !win || !F_mn_dev || !stats_dev)
+        return fail(TCW_ERR_INVALID, "tcw_map_noalloc: null argument");
+    char *base = nullptr;
+    int rc = noalloc_scratch_ptr(h->device, &base);
+    if (rc) return rc;
+    h->noalloc.arena = base;
+    h->noalloc.arena_bytes = TCW_NOALLOC_SCRATCH_BYTES;
+    hipStream_t st = (hipStream_t)stream;
+    // the plan's buffers become views into the scratch (plan_finish), or the
+    // call fails before anything is queued when the map needs more
+    rc = plan_single(h, h->noalloc, atoms, win, flags, st, "tcw_map_noalloc");
+    if (rc) return rc;
+    // every scratch byte a kernel reads is written earlier in the same call
+    // (the stage kernel also clears the single-atom flag), so nothing is cleared
+    return launch_all(h->noalloc, atoms->ts, atoms->cols, atoms->col_stride, F_mn_dev,
+                      stats_dev, st);
 }
 
 int tcw_merge_batch(tcw_handle *h, const tcw_batch_desc *d, const uint32_t *ts,
