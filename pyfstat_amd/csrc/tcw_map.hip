@@ -55,7 +55,6 @@ constexpr double COND_MAX = 1e4;  // rect.cu:104
 // cond < K  <=>  T^2 < Q*D  with T = A+B, D = AB-C^2 > 0, Q = (K+1)^2/K, because
 // cond = (T+s)/(T-s), s^2 = T^2-4D, and (1+cond)^2/cond = T^2/D grows with cond.
 constexpr double QCOND = (COND_MAX + 1.0) * (COND_MAX + 1.0) / COND_MAX;
-constexpr float QCONDF = (float)QCOND;
 
 enum Mode : int32_t {
     MODE_RECT_FAST = 0,     // regular grid: dt0 == dtau == a*TAtom, no wrap-around
@@ -406,10 +405,9 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
 // per-cell F
 // ----------------------------------------------------------------------------
 
-// F from window sums: determinant, condition guard and normalisation in FP64
-// (rect.cu:98-118 / exp.cu:109-129; F = 2 when the guard fails).  The final
-// quotient uses the hardware FP64 reciprocal: its error is far below the
-// float32 rounding of the stored F.
+// F from window sums: determinant, numerator and condition guard in FP64
+// (rect.cu:98-118 / exp.cu:109-129; F = 2 when the guard fails); the final
+// quotient in float32 (the stored map is float32).
 __device__ __forceinline__ float cell_F(double A, double B, double C, double far,
                                         double fai, double fbr, double fbi) {
     const double D = fma(A, B, -(C * C));
@@ -421,13 +419,11 @@ __device__ __forceinline__ float cell_F(double A, double B, double C, double far
     // the map is float32: the quotient of the FP64 numerator and determinant
     // (each rounded to float) through v_rcp_f32 is within ~3e-7 relative, and
     // v_rcp_f64 issues at ~1/3.5 the rate of an FP64 FMA.  The condition test
-    // runs on the same float D (and float T): it differs from an FP64 test only
-    // for windows within ~1e-7 of cond = 1e4.  A, B >= 0 (sums of squares), so
-    // T <= 0 only for an empty window, where D = 0 and the test fails as well.
-    const float Df = (float)D;
-    const float Tf = (float)T;
-    const bool ok = Tf * Tf < QCONDF * Df;
-    const float F = (float)num * __builtin_amdgcn_rcpf(Df);
+    // is FP64 (two multiplies and a compare: one conversion fewer than on
+    // float T and D).  A, B >= 0 (sums of squares), so T <= 0 only for an
+    // empty window, where D = 0 and the test fails as well.
+    const bool ok = T * T < QCOND * D;
+    const float F = (float)num * __builtin_amdgcn_rcpf((float)D);
     return ok ? F : 2.0f;
 }
 
@@ -638,7 +634,7 @@ struct Reducer {
 #pragma unroll
             for (int c = 0; c < NCOLS; c++) {
                 const float e = __builtin_amdgcn_exp2f(fmaf(F[r][c], LOG2E, -ML));
-                sr[r] += e;
+                sr[r] = c == 0 ? e : sr[r] + e;
                 Scol[c] += e;
             }
             Stot += sr[r];

@@ -423,24 +423,3 @@ def test_batched_mixed_windows_multi_detector(hip_ready):
         om = oracle_for(multis[k], wins[k])
         compare_maps(maps[k].F_mn, om)
         assert abs(maps[k].lnBtSG - om.lnBtSG) <= lnb_tol(om.maxF)
-
-
-def test_scale_invariance_full_size(hip_ready):
-    """At the 2000 x 2000 exp size nothing the oracle can afford in a test
-    remains, so check a size-independent property: scaling a2,b2,ab by s^2 and
-    Fa,Fb by s leaves F unchanged."""
-    from pyfstat_amd import tcw_fstat_map_funcs as tcw
-    from pyfstat_amd.atoms import FstatAtomVector, MultiFstatAtomVector
-
-    multi, win = synth.workload_rect_plain(n_sft=4000, n_t0=2000, n_tau=2000)
-    fm = tcw.hip_compute_transient_fstat_map(multi, win, BtSG=True)
-    s = np.float32(4.0)  # power of two: the scaled atoms are exact
-    v = multi.data[0]
-    scaled = MultiFstatAtomVector(1)
-    scaled.data[0] = FstatAtomVector.from_arrays(v.TAtom, v.timestamp, v.a2_alpha * s * s,
-                                                 v.b2_alpha * s * s, v.ab_alpha * s * s,
-                                                 v.Fa_alpha * s, v.Fb_alpha * s)
-    fs = tcw.hip_compute_transient_fstat_map(scaled, win, BtSG=True)
-    rel = np.abs(fs.F_mn - fm.F_mn) / np.maximum(np.abs(fm.F_mn), 1e-30)
-    assert rel.max() < 1e-5
-    assert (fs.t0_ML, fs.tau_ML) == (fm.t0_ML, fm.tau_ML)
