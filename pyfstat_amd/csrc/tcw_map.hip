@@ -85,7 +85,8 @@ struct TmplDev {
     int32_t q0, q00, a;          // MODE_RECT_FAST index maths; a also MODE_EXP_FAST
     int32_t k00, L0, Ls, H;      // MODE_EXP_FAST: first start atom, window lengths
                                  // L(n) = L0 + Ls*n atoms, atoms [0, H) reachable
-    int32_t n_seg, pad_seg;      // MODE_EXP_FAST: row chunks (= n_rowblk)
+    int32_t n_seg;               // MODE_EXP_FAST: row chunks (>= n_rowblk)
+    int32_t init_empty;          // MODE_EXP_FAST: the top chunk's direct sum is empty
     int64_t etab_off;            // MODE_EXP_FAST: doubles, [n_seg + INIT_PARTS][7][N_tau]
     uint32_t o_t0, o_t0Band, o_tau, o_tauBand;  // caller's window (posterior modes)
 };
@@ -1337,16 +1338,81 @@ __global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict_
     }
 }
 
-// Per tau column: E at every chunk top.  Work items (t, -, cb).
-// One 64-thread workgroup per quarter of a column block: the walk is a chain
-// of memory operations per column, and small batches (one large map) have few
-// column blocks, so narrow workgroups spread them over 4x the CUs.
+// Per tau column: E at every chunk top.  Work items (t, -, cb); one wave per
+// column, 4 columns per workgroup.  With f_s(x) = mult x + Y(s) (mult = the
+// weight factor of one chunk of rows), the chunk tops are V(n_seg-1) = the
+// direct sum and V(s-1) = f_s(V(s)), a first-order linear recurrence with a
+// constant coefficient: lane l of a 64-chunk tile holds Y(s_hi - l) and a
+// log-depth (Kogge-Stone) scan, x_l += mult^d x_{l-d} for d = 1 .. 32, gives
+// V(s_hi - l) = mult^l V(s_hi) + x_{l-1}; tiles run from the top chunk down,
+// each carrying V(s_hi - 64) = mult^64 V(s_hi) + x_63.  Slot s holds Y(s) on
+// entry and V(s) on exit (slot 0 has no Y).
+constexpr int SCAN_COLS = WG / 64;  // columns per workgroup
+
+__global__ __launch_bounds__(WG) void exp_scan_kernel(const TmplDev *__restrict__ tds,
+                                                      const ItemDev *__restrict__ items,
+                                                      double *__restrict__ etab) {
+    const ItemDev it = items[blockIdx.x / (TILE_COLS / SCAN_COLS)];
+    const TmplDev td = tds[it.t];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int n = it.cb * TILE_COLS + (int)(blockIdx.x % (TILE_COLS / SCAN_COLS)) * SCAN_COLS + wave;
+    if (n >= (int)td.N_tau) return;  // wave-uniform
+    const double tau = (double)td.tau + (double)n * (double)td.dtau;
+    const double xstep = (double)td.TAtom * (double)td.a * (double)td.rb_rows / tau;
+    // mult[k]^d for the scan steps d = 1, 2, ..., 32 (k = 0: A, B, C; k = 1: Fa, Fb)
+    double pw[2][6];
+    pw[1][0] = exp(-xstep);
+    pw[0][0] = pw[1][0] * pw[1][0];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int d = 1; d < 6; d++) pw[k][d] = pw[k][d - 1] * pw[k][d - 1];
+    // mult^l for this lane, and mult^64
+    double ml[2], m64[2];
+    ml[1] = exp(-xstep * (double)lane);
+    ml[0] = ml[1] * ml[1];
+    m64[0] = pw[0][5] * pw[0][5];
+    m64[1] = pw[1][5] * pw[1][5];
+    double carry[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {  // top chunk: the direct sum's parts
+        carry[c] = 0.0;
+        if (!td.init_empty)
+            for (int pp = 0; pp < INIT_PARTS; pp++)
+                carry[c] += expf_etab(etab, td, td.n_seg + pp, c)[n];
+    }
+    for (int s_hi = td.n_seg - 1; s_hi >= 0; s_hi -= 64) {
+        const int sl = s_hi - lane;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int k = c < 3 ? 0 : 1;
+            double *slot = expf_etab(etab, td, max(sl, 0), c) + n;
+            double x = sl >= 1 ? *slot : 0.0;
+#pragma unroll
+            for (int d = 0; d < 6; d++) {
+                const double y = __shfl_up(x, 1 << d);
+                if (lane >= (1 << d)) x = fma(pw[k][d], y, x);
+            }
+            double sx = __shfl_up(x, 1);
+            if (lane == 0) sx = 0.0;
+            if (sl >= 0) *slot = fma(ml[k], carry[c], sx);
+            const double x63 = __shfl(x, 63);
+            carry[c] = fma(m64[k], carry[c], x63);
+        }
+    }
+}
+
+// The same scan, sequential along the chunks: one thread per tau column (its
+// loads and stores coalesce across the columns).  For batches of many
+// templates with few chunks each, where one wave per column would leave most
+// lanes idle.
 constexpr int SCAN_WG = 64;
 constexpr int SCAN_SPLIT = TILE_COLS / SCAN_WG;
 
-__global__ __launch_bounds__(SCAN_WG) void exp_scan_kernel(const TmplDev *__restrict__ tds,
-                                                           const ItemDev *__restrict__ items,
-                                                           double *__restrict__ etab) {
+__global__ __launch_bounds__(SCAN_WG) void exp_scan_seq_kernel(const TmplDev *__restrict__ tds,
+                                                               const ItemDev *__restrict__ items,
+                                                               double *__restrict__ etab) {
     const ItemDev it = items[blockIdx.x / SCAN_SPLIT];
     const TmplDev td = tds[it.t];
     const int n = it.cb * TILE_COLS + (int)(blockIdx.x % SCAN_SPLIT) * SCAN_WG + threadIdx.x;
@@ -1359,7 +1425,8 @@ __global__ __launch_bounds__(SCAN_WG) void exp_scan_kernel(const TmplDev *__rest
 #pragma unroll
     for (int c = 0; c < NC; c++) {  // top chunk: the direct sum's parts
         E[c] = 0.0;
-        for (int pp = 0; pp < INIT_PARTS; pp++) E[c] += expf_etab(etab, td, td.n_seg + pp, c)[n];
+        if (!td.init_empty)
+            for (int pp = 0; pp < INIT_PARTS; pp++) E[c] += expf_etab(etab, td, td.n_seg + pp, c)[n];
     }
     for (int s = td.n_seg - 1; s >= 1; s--) {
 #pragma unroll
@@ -1784,6 +1851,8 @@ struct tcw_plan {
     std::vector<ItemDev> col_items;  // MODE_EXP_FAST (t, -, column block)
     std::vector<int32_t> exp_tmpls;  // templates needing the FP64 atom table
     int64_t exp_max_N = 0;
+    bool exp_init_needed = false;  // some exp template has a non-empty direct sum
+    bool exp_scan_wave = false;    // the wave-per-column chunk scan (few, long columns)
     int64_t mode_begin[MODE_COUNT + 1] = {0, 0, 0, 0, 0};
     int64_t prefix_doubles = 0, merged_doubles = 0, rowpart_n = 0, colpart_n = 0;
     int64_t etab_doubles = 0;
@@ -1812,6 +1881,8 @@ struct tcw_plan {
         col_items.clear();
         exp_tmpls.clear();
         exp_max_N = 0;
+        exp_init_needed = false;
+        exp_scan_wave = false;
         prefix_doubles = merged_doubles = rowpart_n = colpart_n = F_total = 0;
         etab_doubles = 0;
         max_lds_bytes = 0;
@@ -2029,6 +2100,15 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
                 nseg = std::max<int64_t>(nseg, (std::max<int64_t>(steps, 0) + rb) / rb);
             }
             td.n_seg = (int32_t)nseg;
+            // the top chunk's top row lies at or beyond H: its direct sum is
+            // empty, exp_scan starts from 0 and exp_init need not run
+            td.init_empty = (int64_t)td.H - ((int64_t)td.k00 + (int64_t)td.a * (nseg * rb - 1)) <= 0;
+            if (!td.init_empty) p.exp_init_needed = true;
+            // few (template, column block) pairs, each with many chunks: a
+            // wave per column scans the chunks in log depth; otherwise a
+            // thread per column walks them (coalesced, no idle lanes)
+            if (p.T * (int64_t)td.n_colblk < DIRECT_INIT_MIN_BLOCKS && nseg >= 16)
+                p.exp_scan_wave = true;
             td.etab_off = p.etab_doubles;
             p.etab_doubles += (nseg + INIT_PARTS) * NC * N_tau;
             for (int64_t sg = 1; sg < nseg; sg++)
@@ -2247,7 +2327,7 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     }
     // exp_init and exp_seg read the atom table and write different slots of the
     // E table: with enough blocks for both, exp_init runs on the side stream
-    const int64_t init_blocks = (int64_t)p.col_items.size() * INIT_PARTS;
+    const int64_t init_blocks = p.exp_init_needed ? (int64_t)p.col_items.size() * INIT_PARTS : 0;
     const bool fork = p.side && init_blocks > 0 && !p.seg_items.empty() &&
                       init_blocks + (int64_t)p.seg_items.size() >= EXP_FORK_MIN_BLOCKS;
     if (fork) {
@@ -2268,9 +2348,14 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
         TCW_HIP_TRY(hipEventRecord(p.ev_join, p.side));
         TCW_HIP_TRY(hipStreamWaitEvent(st, p.ev_join, 0));
     }
-    if (!p.col_items.empty())
-        hipLaunchKernelGGL(exp_scan_kernel, dim3((unsigned)(p.col_items.size() * SCAN_SPLIT)),
-                           dim3(SCAN_WG), 0, st,
+    if (!p.col_items.empty() && !p.exp_scan_wave)
+        hipLaunchKernelGGL(exp_scan_seq_kernel, dim3((unsigned)(p.col_items.size() * SCAN_SPLIT)),
+                           dim3(SCAN_WG), 0, st, (const TmplDev *)p.d_tmpl.p,
+                           (const ItemDev *)p.d_col_items.p, (double *)p.d_etab.p);
+    if (!p.col_items.empty() && p.exp_scan_wave)
+        hipLaunchKernelGGL(exp_scan_kernel,
+                           dim3((unsigned)(p.col_items.size() * (TILE_COLS / SCAN_COLS))),
+                           dim3(WG), 0, st,
                            (const TmplDev *)p.d_tmpl.p, (const ItemDev *)p.d_col_items.p,
                            (double *)p.d_etab.p);
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[1], st));
