@@ -28,7 +28,7 @@ CANDIDATES = {
     _hiplib.MODE_RECT_FAST: (16, 32, 64, 128, 192, 256, 384),
     _hiplib.MODE_RECT_GENERIC: (16, 32, 64),
     _hiplib.MODE_EXP_DIRECT: (16, 32, 64),
-    _hiplib.MODE_EXP_FAST: (32, 64, 96, 128),
+    _hiplib.MODE_EXP_FAST: (64, 128, 256, 512, 1024),
 }
 
 def cache_path():

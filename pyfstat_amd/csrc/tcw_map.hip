@@ -292,9 +292,15 @@ __global__ __launch_bounds__(SR_WG) void stage_kernel(
     if (tid < NC) P[(int64_t)N * NC + tid] = s_carry[tid];
 }
 
-// Exponential-window templates: the merged atoms as the FP64 table [N][8]
-// (7 columns + padding), the same float32 detector-order sums as
-// stage_kernel.  A workgroup owns 256 * SXB consecutive bins of one template:
+// Bins of a template's FP64 atom table: the recurrence path reads none at or
+// past H (the first atom no window reaches), the direct path any of the N.
+__host__ __device__ __forceinline__ uint32_t exp_table_bins(const TmplDev &td) {
+    return td.mode == MODE_EXP_FAST ? (uint32_t)td.H : td.N;
+}
+
+// Exponential-window templates: the merged atoms as the FP64 table
+// [exp_table_bins][8] (7 columns + padding), the same float32 detector-order
+// sums as stage_kernel.  A workgroup owns 256 * SXB consecutive bins of one template:
 // per detector, two binary searches bound the block's atoms, which are copied
 // into LDS in chunks with coalesced loads; each thread then sums its SXB bins
 // from LDS (atoms in order, detectors in order).  Grid: (bin blocks, exp
@@ -326,8 +332,9 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
         constexpr int SX_BINS = SXB;
         constexpr int SX_BINS_BLOCK = 256 * SXB;
         const uint32_t B0 = blockIdx.x * (uint32_t)SX_BINS_BLOCK;
-        if (B0 >= td.N) continue;  // block-uniform
-        const uint32_t B1 = min(td.N, B0 + (uint32_t)SX_BINS_BLOCK);
+        const uint32_t NB = exp_table_bins(td);
+        if (B0 >= NB) continue;  // block-uniform
+        const uint32_t B1 = min(NB, B0 + (uint32_t)SX_BINS_BLOCK);
         const uint64_t TA = td.TAtom;
         const uint64_t tb_lo = (uint64_t)td.t_min + (uint64_t)B0 * TA;
         const uint64_t tb_hi = (uint64_t)td.t_min + (uint64_t)B1 * TA;
@@ -1083,144 +1090,165 @@ __device__ __forceinline__ ExpCol exp_col(const TmplDev &td, int n) {
     return ec;
 }
 
-constexpr int MAX_RB_EXPF = 128;
+// Rows per work item of the recurrence at most (the walk stages its atoms in
+// passes, so this bounds the item's pass bookkeeping, not its LDS).
+constexpr int MAX_RB_EXPF = 1024;
 // batches with fewer (template, column block) pairs than this walk the rows
 // above the map instead of summing the top chunk directly
 constexpr int64_t DIRECT_INIT_MIN_BLOCKS = 256;
+// Untuned batches with at least DIRECT_INIT_MIN_BLOCKS pairs: row chunks per
+// column block so that the map kernel has about this many work items.  When
+// the pairs alone reach it, every column is one chunk: the map kernel walks
+// it from the direct sum at its top and no chunk walk from zero (exp_seg) or
+// chunk scan is needed.
+constexpr int64_t EXP_TARGET_ITEMS = 512;
 constexpr int INIT_PARTS = 8;  // blocks sharing one column block's direct sum
 // exp_init beside exp_seg on a second stream only when the two launches hold
 // at least this many blocks together (a fork and join costs ~10 µs)
 constexpr int64_t EXP_FORK_MIN_BLOCKS = 1024;
-// Atoms a walk stages in LDS.  Entering (head) atoms are the same for every
-// lane of a row step: FP64 planes of EXPF_HEAD_CAP, read as broadcasts with
-// no conversion.  Leaving (tail) atoms differ per lane (L grows with the
-// column): float32 planes of EXPF_TAIL_CAP (the merged atoms are float32
-// sums, so the copy is exact), half the bytes.  7 x (960 x 4 + 128 x 8) B =
-// 33.3 KB, so four workgroups share a CU.  Plane strides are multiples of 64
-// elements, so two planes are one ds_read2st64.
-constexpr int EXPF_TAIL_CAP = 960;
-constexpr int EXPF_HEAD_CAP = MAX_RB_EXPF;
-constexpr int EXPF_LDS_BYTES = NC * (EXPF_TAIL_CAP * (int)sizeof(float) +
-                                     EXPF_HEAD_CAP * (int)sizeof(double));
 
-// Atoms a walk over `rows` rows touches: entering atoms [k(r_bot), k(r_top))
-// (nhead = a (rows - 1)) and leaving atoms shifted by L(n), n in [col0,
-// col_last] (ntail = nhead + Ls (cols - 1)).
-__host__ __device__ __forceinline__ bool expf_fits_lds(int a, int rows, int Ls, int cols) {
-    const int nhead = a * (rows - 1);
-    return nhead <= EXPF_HEAD_CAP && nhead + Ls * (cols - 1) <= EXPF_TAIL_CAP;
+// Atoms a walk stages in LDS, per pass of up to EXPW_PASS rows (dt0 == TAtom).
+// Entering (head) atoms are the same on every lane of a row step: atom-major
+// [j][8] doubles, read as broadcasts.  Leaving (tail) atoms differ per lane (L
+// grows with the column): component planes [c][EXPW_TAIL_CAP] doubles, lanes
+// Ls doubles apart (Ls = 3 for dtau = TAtom: 16 lanes hit 32 distinct banks).
+// FP64 in LDS spares the walk a conversion per component and step.  Both
+// ranges start one atom below the pass's last step, so every row reads the
+// next row's atoms ahead (in range) while its own F is evaluated.
+// 896 x 7 x 8 + 136 x 64 B = 58.9 KB: two workgroups per CU.
+constexpr int EXPW_PASS = 128;
+constexpr int EXPW_TAIL_CAP = 896;
+constexpr int EXPW_HEAD_CAP = EXPW_PASS + 8;
+constexpr int EXPF_LDS_BYTES = (NC * EXPW_TAIL_CAP + EXPW_HEAD_CAP * MERGED_STRIDE) *
+                               (int)sizeof(double);
+constexpr int EXPW_MAX_PASSES = MAX_RB_EXPF / 16;
+
+// Rows per LDS pass of a walk over `cols` columns, or 0 when its atoms do not
+// fit (dt0 > TAtom, or windows spread too far): the walk then reads the
+// table directly.  A multiple of 4, at least 16.
+__host__ __device__ __forceinline__ int expw_pass_rows(int a, int Ls, int cols) {
+    if (a != 1) return 0;
+    const int64_t room = (int64_t)EXPW_TAIL_CAP - 1 - (int64_t)Ls * (cols - 1);
+    const int64_t pr = (room < EXPW_PASS ? room : (int64_t)EXPW_PASS) & ~(int64_t)3;
+    return pr >= 16 ? (int)pr : 0;
 }
 
 struct ExpWalk {
-    const double *X;
-    float *lds;     // tail planes [c][EXPF_TAIL_CAP]
-    double *ldsh;   // head planes [c][EXPF_HEAD_CAP]
-    int64_t head_lo, tail_lo;
-    int nhead, ntail;
+    const double *X;  // the template's FP64 atom table
+    double *tl;       // LDS tail planes [c][EXPW_TAIL_CAP]
+    double *hd;       // LDS head atoms [j][MERGED_STRIDE]
+    int64_t head_lo;  // atom in head slot 0
+    int64_t tail_lo;  // atom in tail slot 0
     int H;
-    bool in_lds;
 };
 
-// lds_ok: the launch carries the EXPF_LDS_BYTES of dynamic LDS
-__device__ __forceinline__ ExpWalk expf_stage(const TmplDev &td, const double *X, float *lds,
-                                              int lds_ok, int r_top, int r_bot, int col0,
-                                              int col_last) {
+__device__ __forceinline__ ExpWalk expw_init(const TmplDev &td, const double *X, double *lds) {
     ExpWalk wk;
     wk.X = X;
-    wk.lds = lds;
-    wk.ldsh = reinterpret_cast<double *>(lds + NC * EXPF_TAIL_CAP);
+    wk.tl = lds;
+    wk.hd = lds + NC * EXPW_TAIL_CAP;
+    wk.head_lo = 0;
+    wk.tail_lo = 0;
     wk.H = td.H;
-    wk.head_lo = (int64_t)td.k00 + (int64_t)td.a * r_bot;
-    wk.nhead = td.a * (r_top - r_bot);
-    wk.tail_lo = wk.head_lo + td.L0 + (int64_t)td.Ls * col0;
-    wk.ntail = wk.nhead + td.Ls * (col_last - col0);
-    wk.in_lds = lds_ok && wk.nhead <= EXPF_HEAD_CAP && wk.ntail <= EXPF_TAIL_CAP;  // uniform
-    if (wk.in_lds) {
-        for (int idx = threadIdx.x; idx < wk.nhead * NC; idx += WG) {
-            const int j = idx / NC;
-            const int c = idx - j * NC;
-            const int64_t atom = wk.head_lo + j;
-            wk.ldsh[c * EXPF_HEAD_CAP + j] = atom < wk.H ? X[atom * MERGED_STRIDE + c] : 0.0;
-        }
-        for (int idx = threadIdx.x; idx < wk.ntail * NC; idx += WG) {
-            const int j = idx / NC;
-            const int c = idx - j * NC;
-            const int64_t atom = wk.tail_lo + j;
-            lds[c * EXPF_TAIL_CAP + j] = atom < wk.H ? (float)X[atom * MERGED_STRIDE + c] : 0.0f;
-        }
-    }
-    __syncthreads();
     return wk;
 }
 
-// One row step: k -> k - a (a atom steps of the recurrence).
-__device__ __forceinline__ void expf_step(const ExpWalk &wk, const ExpCol &ec, int a, int k,
-                                          double (&E)[NC]) {
+// Staging of one pass: the atoms of the steps at rows sb..st (dt0 == TAtom)
+// of columns col0..col_last -- head atoms k(m) - 1 and tail atoms k(m) - 1 +
+// L(n), each range with one more atom below (read ahead at row sb); atoms
+// outside [0, H) count as zero.  Thread tid moves component tid & 7 of atoms
+// (tid >> 3) + 32 k (consecutive threads read consecutive doubles of the
+// table), through registers: expw_fetch issues the loads, expw_put writes
+// LDS, so the next pass's loads stay in flight during the current walk.
+constexpr int EXPW_HEAD_PER = (EXPW_HEAD_CAP + 31) / 32;
+constexpr int EXPW_TAIL_PER = EXPW_TAIL_CAP / 32;
+static_assert(EXPW_TAIL_CAP % 32 == 0 && WG == 256, "staging decomposition");
+
+struct ExpPass {
+    int64_t head_lo, tail_lo;
+    int nhead, ntail;
+    double h[EXPW_HEAD_PER];
+    double t[EXPW_TAIL_PER];
+};
+
+__device__ __forceinline__ void expw_fetch(const ExpWalk &wk, const TmplDev &td, int st, int sb,
+                                           int col0, int col_last, ExpPass &ps) {
+    ps.head_lo = (int64_t)td.k00 + sb - 2;
+    ps.tail_lo = ps.head_lo + td.L0 + (int64_t)td.Ls * col0;
+    ps.nhead = st - sb + 2;
+    ps.ntail = ps.nhead + td.Ls * (col_last - col0);
+    const int c = threadIdx.x & 7, j0 = threadIdx.x >> 3;
+#pragma unroll
+    for (int k = 0; k < EXPW_HEAD_PER; k++) {
+        const int j = j0 + 32 * k;
+        const int64_t atom = ps.head_lo + j;
+        ps.h[k] = j < ps.nhead && atom >= 0 && atom < wk.H ? wk.X[atom * MERGED_STRIDE + c] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < EXPW_TAIL_PER; k++) {
+        const int j = j0 + 32 * k;
+        const int64_t atom = ps.tail_lo + j;
+        ps.t[k] = c < NC && j < ps.ntail && atom >= 0 && atom < wk.H
+                      ? wk.X[atom * MERGED_STRIDE + c]
+                      : 0.0;
+    }
+}
+
+// The caller synchronises before the walk reads LDS.
+__device__ __forceinline__ void expw_put(ExpWalk &wk, const ExpPass &ps) {
+    wk.head_lo = ps.head_lo;
+    wk.tail_lo = ps.tail_lo;
+    const int c = threadIdx.x & 7, j0 = threadIdx.x >> 3;
+#pragma unroll
+    for (int k = 0; k < EXPW_HEAD_PER; k++) {
+        const int j = j0 + 32 * k;
+        if (j < ps.nhead) wk.hd[j * MERGED_STRIDE + c] = ps.h[k];
+    }
+#pragma unroll
+    for (int k = 0; k < EXPW_TAIL_PER; k++) {
+        const int j = j0 + 32 * k;
+        if (c < NC && j < ps.ntail) wk.tl[c * EXPW_TAIL_CAP + j] = ps.t[k];
+    }
+}
+
+// Operands of one row step from LDS: head slot jh (wave-uniform), tail slot jt.
+__device__ __forceinline__ void expw_load(const ExpWalk &wk, int jh, int jt, double (&xh)[NC],
+                                          double (&xt)[NC]) {
+    const double *h = wk.hd + jh * MERGED_STRIDE;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        xh[c] = h[c];
+        xt[c] = wk.tl[c * EXPW_TAIL_CAP + jt];
+    }
+}
+
+// E(k - 1) = rho E(k) + (x_{k-1} - rho^L x_{k-1+L})
+__device__ __forceinline__ void expw_step(const ExpCol &ec, const double (&xh)[NC],
+                                          const double (&xt)[NC], double (&E)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int g = c < 3 ? 0 : 1;
+        const double d = fma(-ec.rhoL[g], xt[c], xh[c]);
+        E[c] = fma(ec.rho[g], E[c], d);
+    }
+}
+
+// One row step read from the table: k -> k - a (a atom steps).  The entering
+// atoms k - l >= 0 for every step a walk takes.
+__device__ __forceinline__ void expw_step_table(const ExpWalk &wk, const ExpCol &ec, int a,
+                                                int64_t k, double (&E)[NC]) {
     for (int l = 1; l <= a; l++) {
-        const int64_t kk = (int64_t)k - l;  // entering atom (wave-uniform)
-        const int64_t kt = kk + ec.L;       // leaving atom
+        const int64_t kk = k - l;     // entering atom (wave-uniform)
+        const int64_t kt = kk + ec.L;  // leaving atom
+        const double *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
+        const double *pt = wk.X + min(kt, (int64_t)wk.H - 1) * MERGED_STRIDE;
         double xh[NC], xt[NC];
-        if (wk.in_lds) {
-            const int jh = (int)(kk - wk.head_lo);
-            const int jt = (int)(kt - wk.tail_lo);
-#pragma unroll
-            for (int c = 0; c < NC; c++) {
-                xh[c] = wk.ldsh[c * EXPF_HEAD_CAP + jh];
-                xt[c] = (double)wk.lds[c * EXPF_TAIL_CAP + jt];
-            }
-        } else {
-            const double *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
-            const double *pt = wk.X + min(kt, (int64_t)wk.H - 1) * MERGED_STRIDE;
-#pragma unroll
-            for (int c = 0; c < NC; c++) {
-                xh[c] = kk < wk.H ? ph[c] : 0.0;
-                xt[c] = kt < wk.H ? pt[c] : 0.0;
-            }
-        }
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const int g = c < 3 ? 0 : 1;
-            E[c] = fma(-ec.rhoL[g], xt[c], fma(ec.rho[g], E[c], xh[c]));
+            xh[c] = kk < wk.H ? ph[c] : 0.0;
+            xt[c] = kt < wk.H ? pt[c] : 0.0;
         }
-    }
-}
-
-// The common case of one row step = one atom step (dt0 == TAtom) with the
-// atoms staged in LDS: jh / jt are the head and tail LDS slots of the
-// entering atom (k - 1) and of the leaving atom (k - 1 + L), both falling by
-// one per row.
-__device__ __forceinline__ void expf_step1(const ExpWalk &wk, const ExpCol &ec, int jh, int jt,
-                                           double (&E)[NC]) {
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const int g = c < 3 ? 0 : 1;
-        const double xh = wk.ldsh[c * EXPF_HEAD_CAP + jh];
-        const double xt = (double)wk.lds[c * EXPF_TAIL_CAP + jt];
-        E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh));
-    }
-}
-
-// The same step with the entering atom kh = k - 1 (the same on every lane)
-// read from the FP64 table through scalar loads: an SGPR operand of the FMA,
-// no LDS read or conversion.  Used by the walk that evaluates no F
-// (exp_seg_kernel); in the map kernel the scalar loads would share the LGKM
-// wait counter with its LDS reads and serialise them (measured 11% slower).
-__device__ __forceinline__ void expf_step1_sh(const ExpWalk &wk, const ExpCol &ec, int64_t kh,
-                                              int jt, double (&E)[NC]) {
-    double xh[NC];
-    if (kh < wk.H) {  // wave-uniform; atoms at or past H count as zero
-        const double *__restrict__ ph = wk.X + kh * MERGED_STRIDE;
-#pragma unroll
-        for (int c = 0; c < NC; c++) xh[c] = ph[c];
-    } else {
-#pragma unroll
-        for (int c = 0; c < NC; c++) xh[c] = 0.0;
-    }
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const int g = c < 3 ? 0 : 1;
-        const double xt = (double)wk.lds[c * EXPF_TAIL_CAP + jt];
-        E[c] = fma(-ec.rhoL[g], xt, fma(ec.rho[g], E[c], xh[c]));
+        expw_step(ec, xh, xt, E);
     }
 }
 
@@ -1231,11 +1259,10 @@ __device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, in
 
 // Y(s), s >= 1: the recurrence across chunk s (from its top row down to the
 // top row of chunk s-1) started from zero.  Work items (t, s, cb).
-__global__ __launch_bounds__(WG) void exp_seg_kernel(
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void exp_seg_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items,
     const double *__restrict__ merged, double *__restrict__ etab, int lds_ok) {
     extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
-    float *lds = reinterpret_cast<float *>(lds_raw);  // staged atoms, exact float32
     const ItemDev it = items[block_slot()];
     const TmplDev td = tds[it.t];
     const int RB = td.rb_rows;
@@ -1244,26 +1271,47 @@ __global__ __launch_bounds__(WG) void exp_seg_kernel(
     const int N_tau = (int)td.N_tau;
     const bool valid = n < N_tau;
     const int col_last = min(col0 + TILE_COLS, N_tau) - 1;
+    // steps at rows r_top .. r_bot + 1
     const int r_top = (it.rb + 1) * RB - 1, r_bot = it.rb * RB - 1;
-    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_ok, r_top, r_bot, col0,
-                                  col_last);
-    const ExpCol ec = exp_col(td, valid ? n : col0);
+    const int ncol = valid ? n : col0;  // lanes past the map follow the first column
+    const ExpCol ec = exp_col(td, ncol);
+    ExpWalk wk = expw_init(td, merged + td.merged_off, lds_raw);
     double E[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) E[c] = 0.0;
-    int k = td.k00 + td.a * r_top;
-    if (wk.in_lds && td.a == 1) {  // block-uniform
-        int64_t kh = (int64_t)k - 1;
-        int jt = (int)(k - 1 + ec.L - wk.tail_lo);
+    const int pr = lds_ok ? expw_pass_rows(td.a, td.Ls, col_last - col0 + 1) : 0;  // uniform
+    if (pr > 0) {
+        ExpPass ps;
+        expw_fetch(wk, td, r_top, max(r_bot + 1, r_top - pr + 1), col0, col_last, ps);
+        for (int st = r_top; st > r_bot; st -= pr) {
+            const int sb = max(r_bot + 1, st - pr + 1);
+            expw_put(wk, ps);
+            __syncthreads();
+            if (st - pr > r_bot)  // the next pass's atoms, in flight during this walk
+                expw_fetch(wk, td, st - pr, max(r_bot + 1, st - 2 * pr + 1), col0, col_last, ps);
+            int jh = st - sb + 1;
+            int jt = jh + td.Ls * (ncol - col0);
+            double xh[NC], xt[NC];
+            expw_load(wk, jh, jt, xh, xt);
 #pragma unroll 2
-        for (int m = r_top; m > r_bot; m--) {
-            expf_step1_sh(wk, ec, kh, jt, E);
-            kh--;
-            jt--;
+            for (int m = st; m >= sb; m--) {
+                double nh[NC], nt[NC];
+                expw_load(wk, jh - 1, jt - 1, nh, nt);  // the next row's atoms
+                expw_step(ec, xh, xt, E);
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    xh[c] = nh[c];
+                    xt[c] = nt[c];
+                }
+                jh--;
+                jt--;
+            }
+            __syncthreads();  // the next pass restages
         }
     } else {
+        int64_t k = (int64_t)td.k00 + (int64_t)td.a * r_top;
         for (int m = r_top; m > r_bot; m--) {
-            expf_step(wk, ec, td.a, k, E);
+            expw_step_table(wk, ec, td.a, k, E);
             k -= td.a;
         }
     }
@@ -1441,59 +1489,99 @@ __global__ __launch_bounds__(SCAN_WG) void exp_scan_seq_kernel(const TmplDev *__
     for (int c = 0; c < NC; c++) expf_etab(etab, td, 0, c)[n] = E[c];
 }
 
-// Rows r_top .. row0 of one chunk walk in groups of 4 (RB is a multiple of 16),
-// reduced together.  Rows past the map (m > m_hi, only in a template's last
-// chunk) are evaluated but neither stored nor reduced, which keeps the loop
-// free of branches; only the chunk's final row takes no step.
-template <bool BTSG, bool STRICT, bool STEP1>
-__device__ __forceinline__ void expf_rows(const TmplDev &td, const ExpWalk &wk, const ExpCol &ec,
+// Word 3 of a raw buffer resource: 32-bit data format, as gfx9 raw buffer
+// accesses require.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+
+// Rows st .. sb of a chunk (one pass) in groups of 4 (the pass's rows are a
+// multiple of 4), reduced together.  Rows past the map (m > m_hi, only in a
+// template's last chunk) are evaluated but neither stored nor reduced, which
+// keeps the loop free of branches.  LDS: the pass's atoms are staged (dt0 ==
+// TAtom), jt_off = Ls * (the
+// lane's column - the first column); otherwise each step reads the table.
+template <bool BTSG, bool STRICT, bool LDS>
+__device__ __forceinline__ void expw_rows(const TmplDev &td, const ExpWalk &wk, const ExpCol &ec,
                                           double (&E)[NC], Reducer<BTSG, 1> &red, bool &single,
-                                          float *__restrict__ Ft, int r_top, int row0, int m_hi,
-                                          int n, bool valid, int lane, float *sh_rowM,
-                                          float *sh_rowS) {
+                                          float *__restrict__ Ft, int st, int sb, int row0,
+                                          int m_hi, int n, bool valid, int lane, int jt_off,
+                                          float *sh_rowM, float *sh_rowS) {
     const int N_tau = (int)td.N_tau;
     const int colidx[1] = {n};
-    int k = td.k00 + td.a * r_top;
-    int jh = (int)(k - 1 - wk.head_lo);
-    int jt = (int)(k - 1 + ec.L - wk.tail_lo);
-    for (int g = r_top; g >= row0; g -= 4) {
+    int64_t k = (int64_t)td.k00 + (int64_t)td.a * st;  // first window atom of row m
+    // stores without a branch: one buffer resource over the pass's rows inside
+    // the map, a byte offset per lane that falls outside it for lanes past the
+    // last column (2^31 and below it by less than 2^31 minus the resource's
+    // size, as the plan bounds N_tau); rows past the map lie beyond it too
+    const int nv = min(st, m_hi) - sb + 1;
+    const uint32_t rowb = (uint32_t)N_tau * (uint32_t)sizeof(float);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        Ft + (int64_t)sb * N_tau, (short)0, nv > 0 ? (int)((uint32_t)nv * rowb) : 0,
+        BUF_RSRC_WORD3);
+    uint32_t vo = valid ? (uint32_t)(st - sb) * rowb + (uint32_t)n * (uint32_t)sizeof(float)
+                        : 0x80000000u;  // row st
+    int jh = st - sb + 1;  // LDS path: head slot of row st's step
+    int jt = jh + jt_off;  // and tail slot
+    double xh[NC], xt[NC];
+    if (LDS) expw_load(wk, jh, jt, xh, xt);
+    for (int g = st; g >= sb; g -= 4) {
         const bool last = g - 3 == row0;
-        float F4[4][1];
+        float F4[4][1];  // F4[r] is row g - 3 + r
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int m = g - j;
-            const float fv = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
-            const bool keep = valid && m <= m_hi;
-            if (keep) Ft[(int64_t)m * N_tau + n] = fv;
-            if (STRICT) single |= keep && (k >= (int)td.N - 1);
-            F4[3 - j][0] = keep ? fv : -INFINITY;  // F4[r] is row g - 3 + r
-            if (j < 3 || !last) {
-                if (STEP1) {
-                    expf_step1(wk, ec, jh, jt, E);
-                    jh--;
-                    jt--;
-                } else {
-                    expf_step(wk, ec, td.a, k, E);
+            double nh[NC], nt[NC];
+            if (LDS) expw_load(wk, jh - 1, jt - 1, nh, nt);  // the next row's atoms
+            // every lane evaluates F (no branch around it, so the compiler
+            // can overlap the rows' evaluations); stores and masks follow
+            F4[3 - j][0] = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
+            if (STRICT) single |= valid && m <= m_hi && (k >= (int64_t)td.N - 1);
+            // the chunk's final row takes no step: from the table it could
+            // read below atom 0; from LDS the step reads staged slots and its
+            // result is never used, so it runs unconditionally (no branch)
+            if (LDS)
+                expw_step(ec, xh, xt, E);
+            else if (j < 3 || !last)
+                expw_step_table(wk, ec, td.a, k, E);
+            if (LDS) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    xh[c] = nh[c];
+                    xt[c] = nt[c];
                 }
+                jh--;
+                jt--;
             }
             k -= td.a;
         }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = g - 3 + r;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(F4[r][0]), rs,
+                                                  (int)(vo - (uint32_t)(3 - r) * rowb), 0, 0);
+            F4[r][0] = valid && m <= m_hi ? F4[r][0] : -INFINITY;
+        }
+        vo -= 4 * rowb;
         if (g - 3 <= m_hi)
-            red.template add<4>(F4, g - 3, colidx, N_tau, lane, g - 3 - row0, sh_rowM, sh_rowS);
+            red.template add<4>(F4, g - 3, colidx, N_tau, lane, g - 3 - sb, sh_rowM, sh_rowS);
     }
 }
 
-// The map: chunk rb walks rows (rb+1)*RB - 1 .. rb*RB from E_top(rb).
+// The map: chunk rb walks rows (rb+1)*RB - 1 .. rb*RB from E_top(rb), in
+// passes.  BTSG row sums: after each pass the 4 waves' sums of its rows are
+// combined relative to the pass reference (the waves' largest reference then)
+// and written; at the end, the rows of passes whose reference lies below the
+// item's are rescaled by the threads that wrote them.
 template <bool BTSG, bool STRICT>
-__global__ __launch_bounds__(WG) void exp_fast_kernel(
+// Two workgroups share a CU (LDS): registers for two waves per SIMD.
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void exp_fast_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
     const double *__restrict__ merged, const double *__restrict__ etab,
     float *__restrict__ Fout, ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
     float *__restrict__ colpart, int *__restrict__ flags, int lds_ok) {
     extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
-    float *lds = reinterpret_cast<float *>(lds_raw);  // staged atoms, exact float32
-    __shared__ float sh_rowM[4][MAX_RB_EXPF];
-    __shared__ float sh_rowS[4][MAX_RB_EXPF];
+    __shared__ float sh_rowM[4][EXPW_PASS];
+    __shared__ float sh_rowS[4][EXPW_PASS];
+    __shared__ float sh_passM[EXPW_MAX_PASSES];
     __shared__ float sh_best[4];
     __shared__ int sh_bidx[4];
     __shared__ float sh_M[4];
@@ -1512,9 +1600,9 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     const int n = col0 + threadIdx.x;
     const bool valid = n < N_tau;
     const int col_last = min(col0 + TILE_COLS, N_tau) - 1;
-    const ExpWalk wk = expf_stage(td, merged + td.merged_off, lds, lds_ok, r_top, row0, col0,
-                                  col_last);
-    const ExpCol ec = exp_col(td, valid ? n : col0);
+    const int ncol = valid ? n : col0;
+    const ExpCol ec = exp_col(td, ncol);
+    ExpWalk wk = expw_init(td, merged + td.merged_off, lds_raw);
     double E[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) E[c] = valid ? expf_etab((double *)etab, td, it.rb, c)[n] : 0.0;
@@ -1523,12 +1611,42 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
     red.init();
     bool single = false;
     float *__restrict__ Ft = Fout + td.F_off;
-    if (wk.in_lds && td.a == 1)  // block-uniform
-        expf_rows<BTSG, STRICT, true>(td, wk, ec, E, red, single, Ft, r_top, row0, m_hi, n, valid,
-                                      lane, sh_rowM[wave], sh_rowS[wave]);
-    else
-        expf_rows<BTSG, STRICT, false>(td, wk, ec, E, red, single, Ft, r_top, row0, m_hi, n, valid,
-                                       lane, sh_rowM[wave], sh_rowS[wave]);
+    float *__restrict__ rp = rowpart + td.rowpart_off + (int64_t)it.cb * td.N_t0;
+    const int prl = lds_ok ? expw_pass_rows(td.a, td.Ls, col_last - col0 + 1) : 0;  // uniform
+    const int pr = prl > 0 ? prl : EXPW_PASS;
+    int npass = 0;
+    ExpPass ps;
+    if (prl > 0) expw_fetch(wk, td, r_top, max(row0, r_top - pr + 1), col0, col_last, ps);
+    for (int st = r_top; st >= row0; st -= pr, npass++) {
+        const int sb = max(row0, st - pr + 1);
+        if (prl > 0) {
+            expw_put(wk, ps);
+            __syncthreads();
+            if (st - pr >= row0)  // the next pass's atoms, in flight during this walk
+                expw_fetch(wk, td, st - pr, max(row0, st - 2 * pr + 1), col0, col_last, ps);
+            expw_rows<BTSG, STRICT, true>(td, wk, ec, E, red, single, Ft, st, sb, row0, m_hi, n,
+                                          valid, lane, td.Ls * (ncol - col0), sh_rowM[wave],
+                                          sh_rowS[wave]);
+        } else {
+            expw_rows<BTSG, STRICT, false>(td, wk, ec, E, red, single, Ft, st, sb, row0, m_hi, n,
+                                           valid, lane, 0, sh_rowM[wave], sh_rowS[wave]);
+        }
+        const int nv = min(st, m_hi) - sb + 1;  // rows of the pass inside the map
+        if (BTSG && nv > 0) {  // block-uniform
+            if (lane == 0) sh_M[wave] = red.ML;
+            __syncthreads();
+            const float Ms = fmaxf(fmaxf(sh_M[0], sh_M[1]), fmaxf(sh_M[2], sh_M[3]));
+            for (int rl = threadIdx.x; rl < nv; rl += WG) {
+                float S = 0.0f;
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    if (sh_M[w] != -INFINITY) S += sh_rowS[w][rl] * exp2_diff(sh_rowM[w][rl], Ms);
+                rp[sb + rl] = S;
+            }
+            if (threadIdx.x == 0) sh_passM[npass] = Ms;
+        }
+        __syncthreads();  // staging, row sums and sh_M are rewritten by the next pass
+    }
     if (STRICT && single) atomicOr(flags + it.t, 1);
     // combine: each column belongs to one lane; each row to all 4 waves
     wave_argmax(red.best, red.bidx);
@@ -1546,13 +1664,17 @@ __global__ __launch_bounds__(WG) void exp_fast_kernel(
         if (valid)
             colpart[td.colpart_off + (int64_t)it.rb * td.N_tau + n] =
                 red.ML == -INFINITY ? 0.0f : red.Scol[0] * exp2_diff(red.ML, M);
-        const int nrows = m_hi - row0 + 1;
-        for (int rl = threadIdx.x; rl < nrows; rl += WG) {
-            float S = 0.0f;
-#pragma unroll
-            for (int w = 0; w < 4; w++)
-                if (sh_M[w] != -INFINITY) S += sh_rowS[w][rl] * exp2_diff(sh_rowM[w][rl], M);
-            rowpart[td.rowpart_off + (int64_t)it.cb * td.N_t0 + row0 + rl] = S;
+        // rows written relative to an earlier, lower reference: the same
+        // threads (same rows) rescale them
+        for (int j = 0; j < npass; j++) {
+            const int st = r_top - j * pr;
+            const int sb = max(row0, st - pr + 1);
+            const int nv = min(st, m_hi) - sb + 1;
+            if (nv <= 0) continue;
+            const float Ms = sh_passM[j];
+            if (Ms == M || Ms == -INFINITY) continue;  // block-uniform
+            const float sc = exp2_diff(Ms, M);
+            for (int rl = threadIdx.x; rl < nv; rl += WG) rp[sb + rl] *= sc;
         }
     }
     if (threadIdx.x == 0) {
@@ -2041,7 +2163,10 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         const bool on_grid = w.t0 >= t_min && (w.t0 - t_min) % TAtom == 0 &&
                              w.dt0 % TAtom == 0 && w.tau % TAtom == 0 &&
                              w.dtau % TAtom == 0 && w.tau >= TAtom;
-        if (!(p.flags & TCW_FLAG_GENERIC) && on_grid && end < (1ull << 32)) {
+        // (rows of at most 2^20 cells: the map kernel's row stores address a
+        // pass of rows by 32-bit byte offsets)
+        if (!(p.flags & TCW_FLAG_GENERIC) && on_grid && end < (1ull << 32) &&
+            N_tau <= (int64_t)1 << 20) {
             mode = MODE_EXP_FAST;
             td.k00 = (int32_t)((w.t0 - t_min) / TAtom);
             td.a = (int32_t)(w.dt0 / TAtom);
@@ -2076,17 +2201,25 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         rb = want >= RB_DEFAULT[mode] ? RB_DEFAULT[mode]
                                       : std::max(16, ((int)std::ceil(want) + 15) / 16 * 16);
     }
-    if (mode == MODE_EXP_FAST && req <= 0 &&
-        p.T * ((N_tau + TILE_COLS - 1) / TILE_COLS) < DIRECT_INIT_MIN_BLOCKS)
-        rb = EXP_SMALL_BATCH_ROWS;  // few column blocks: shorter chunks, more of them
+    if (mode == MODE_EXP_FAST && req <= 0) {
+        const int64_t pairs = p.T * ((N_tau + TILE_COLS - 1) / TILE_COLS);
+        if (pairs < DIRECT_INIT_MIN_BLOCKS) {
+            rb = EXP_SMALL_BATCH_ROWS;  // few column blocks: shorter chunks, more of them
+        } else {
+            // about EXP_TARGET_ITEMS work items: one chunk per column when the
+            // batch has that many column blocks
+            const int64_t chunks = std::max<int64_t>(1, (EXP_TARGET_ITEMS + pairs - 1) / pairs);
+            const int64_t per = (N_t0 + chunks - 1) / chunks;
+            rb = (int)std::min<int64_t>(MAX_RB_EXPF, (per + 15) / 16 * 16);
+        }
+    }
+    if (mode == MODE_EXP_FAST)  // rows past the map are walked: no longer chunks than the map
+        rb = (int)std::min<int64_t>(rb, (N_t0 + 15) / 16 * 16);
     td.rb_rows = rb;
     td.n_rowblk = (uint32_t)((N_t0 + rb - 1) / rb);
     td.n_colblk = (uint32_t)((N_tau + TILE_COLS - 1) / TILE_COLS);
     if (mode == MODE_EXP_DIRECT || mode == MODE_EXP_FAST) {
-        td.merged_off = p.merged_doubles;
-        p.merged_doubles += (int64_t)N * MERGED_STRIDE;
-        p.exp_tmpls.push_back((int32_t)t);
-        p.exp_max_N = std::max<int64_t>(p.exp_max_N, N);
+        td.mode = mode;  // exp_table_bins reads it
         if (mode == MODE_EXP_FAST) {
             // E at the top chunk's top row comes from a direct O(L) sum
             // (exp_init_kernel, slot n_seg of the table).  With enough
@@ -2116,9 +2249,14 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
                     p.seg_items.push_back(ItemDev{(int32_t)t, (int32_t)sg, (int32_t)cbk, 0});
             for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
                 p.col_items.push_back(ItemDev{(int32_t)t, 0, (int32_t)cbk, 0});
-            if (expf_fits_lds(td.a, rb + 1, td.Ls, TILE_COLS))  // fixed plane strides
-                p.max_lds_expf = EXPF_LDS_BYTES;
+            if (expw_pass_rows(td.a, td.Ls, (int)std::min<int64_t>(TILE_COLS, N_tau)) > 0)
+                p.max_lds_expf = EXPF_LDS_BYTES;  // fixed plane strides
         }
+        const int64_t nbins = exp_table_bins(td);
+        td.merged_off = p.merged_doubles;
+        p.merged_doubles += nbins * MERGED_STRIDE;
+        p.exp_tmpls.push_back((int32_t)t);
+        p.exp_max_N = std::max<int64_t>(p.exp_max_N, nbins);
     } else {
         td.prefix_off = p.prefix_doubles;
         p.prefix_doubles += ((int64_t)N + 1) * NC;
