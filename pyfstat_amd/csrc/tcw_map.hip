@@ -1067,10 +1067,12 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
 //                    E_top(s-1) = rho^(a RB) E_top(s) + Y(s);
 //   exp_fast_kernel  each chunk walks its rows from E_top, evaluating F.
 // Chunk s spans rows s*RB .. (s+1)*RB - 1 (rows past N_t0 are walked but not
-// output).  Atoms at or past H (no output window reaches them) count as zero
-// everywhere, which keeps the recurrence exact for every output row.  Atoms
-// entering and leaving the windows of a chunk are staged in LDS; one thread
-// per tau column.
+// output).  Batches with many (template, column block) pairs use one chunk per
+// column (RB up to 1024 rows): no exp_seg or exp_scan, and exp_fast adds the
+// direct sum's parts itself.  Atoms at or past H (no output window reaches
+// them) count as zero everywhere, which keeps the recurrence exact for every
+// output row.  Atoms entering and leaving the windows of a chunk are staged
+// in LDS, pass by pass; one thread per tau column.
 
 struct ExpCol {
     double rho[2];   // rho^2 (a2, b2, ab), rho (Fa, Fb)
@@ -1323,66 +1325,145 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
 // E at the top row of the top chunk, r = n_seg*RB - 1, by the direct sum
 // E(k) = sum_{l < L} rho^l x_{k+l} (atoms at or past H count as zero, as in
-// the walks), split over INIT_PARTS blocks per (template, column block): part
-// p sums its share of l into slot n_seg + p; the scan adds the parts.  One
-// lane per tau column.  The l range is block-uniform (from the block's longest
-// window): the block copies its atoms into LDS in chunks with coalesced loads
-// and the lanes read the same row (a broadcast), each up to its own window
-// length.
-constexpr int INIT_CHUNK = 256;  // atoms per LDS chunk: 256 x 64 B = 16 KB
+// the walks), split over INIT_PARTS waves per (template, column block): part
+// p sums its share of the block's l range into slot n_seg + p; the scan adds
+// the parts.  Each lane sums INIT_GROUPS columns (its lane, + 64, + 128, +
+// 192 of the block), so every atom, read from LDS as a broadcast, feeds up to
+// 4 x 9 FMA/MUL instructions.  The wave stages its atoms in LDS in chunks of
+// INIT_CHUNK with coalesced loads, the next chunk in flight in registers
+// while the current one is summed.  The l range runs in segments that end
+// where a column group's longest window ends (groups whose windows have all
+// ended drop out); within 63 Ls atoms of that end the lanes of the group
+// mask their weights.
+constexpr int INIT_WG = 64;
+constexpr int INIT_GROUPS = TILE_COLS / INIT_WG;
+constexpr int INIT_CHUNK = 64;                                // atoms per LDS chunk (4 KB)
+constexpr int INIT_PER = INIT_CHUNK * MERGED_STRIDE / INIT_WG;  // doubles per lane per chunk
 
-__global__ __launch_bounds__(WG) void exp_init_kernel(const TmplDev *__restrict__ tds,
-                                                      const ItemDev *__restrict__ items,
-                                                      const double *__restrict__ merged,
-                                                      double *__restrict__ etab) {
-    __shared__ double s_x[INIT_CHUNK * MERGED_STRIDE];
-    const ItemDev it = items[blockIdx.x / INIT_PARTS];
-    const int part = (int)(blockIdx.x % INIT_PARTS);
+struct InitState {
+    double E[INIT_GROUPS][NC];
+    double w[INIT_GROUPS][2];
+    double rho[INIT_GROUPS][2];
+    int64_t lr[INIT_GROUPS];  // this lane's window end, per group
+    int64_t lo[INIT_GROUPS];  // the group's shortest window (uniform)
+};
+
+template <bool MASK>
+__device__ __forceinline__ void init_add(InitState &S, int i, int64_t l, const double *x) {
+    double wa = S.w[i][0], wb = S.w[i][1];
+    if (MASK) {
+        const bool in = l < S.lr[i];
+        wa = in ? wa : 0.0;
+        wb = in ? wb : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) S.E[i][c] = fma(c < 3 ? wa : wb, x[c], S.E[i][c]);
+    S.w[i][0] *= S.rho[i][0];
+    S.w[i][1] *= S.rho[i][1];
+}
+
+// atoms [a, b) of the chunk starting at atom c0, for the column groups I0 ..
+// INIT_GROUPS - 1; group I0 masks its lanes' weights when MASK0, the later
+// groups never need to (their shortest window reaches past b)
+template <int I0, bool MASK0>
+__device__ __forceinline__ void init_segment(InitState &S, const double *buf, int64_t c0,
+                                             int64_t a, int64_t b) {
+#pragma unroll 2
+    for (int64_t l = a; l < b; l++) {
+        double x[NC];
+        const double *p = buf + (l - c0) * MERGED_STRIDE;  // broadcast
+#pragma unroll
+        for (int c = 0; c < NC; c++) x[c] = p[c];
+        init_add<MASK0>(S, I0, l, x);
+#pragma unroll
+        for (int i = I0 + 1; i < INIT_GROUPS; i++) init_add<false>(S, i, l, x);
+    }
+}
+
+__global__ __launch_bounds__(INIT_WG) void exp_init_kernel(const TmplDev *__restrict__ tds,
+                                                           const ItemDev *__restrict__ items,
+                                                           const double *__restrict__ merged,
+                                                           double *__restrict__ etab) {
+    __shared__ double buf[INIT_CHUNK * MERGED_STRIDE];
+    const ItemDev it = items[blockIdx.x];  // (t, part, column block)
+    const int part = it.rb;
     const TmplDev td = tds[it.t];
     const int col0 = it.cb * TILE_COLS;
     const int col_last = min(col0 + TILE_COLS, (int)td.N_tau) - 1;
-    const int n = col0 + threadIdx.x;
-    const bool valid = n <= col_last;
-    const ExpCol ec = exp_col(td, valid ? n : col_last);
+    const int lane = (int)threadIdx.x;
     const int64_t k0 = (int64_t)td.k00 + (int64_t)td.a * ((int64_t)td.n_seg * td.rb_rows - 1);
     const int64_t room = (int64_t)td.H - k0;
+    auto clip = [&](int64_t L) { return max(min(L, room), (int64_t)0); };
     // longest window of the block (L grows with n), clipped at H: uniform
-    const int64_t lr_blk = max(min((int64_t)td.L0 + (int64_t)td.Ls * col_last, room), (int64_t)0);
-    const int64_t lr = max(min((int64_t)ec.L, room), (int64_t)0);  // this lane
+    const int64_t lr_blk = clip((int64_t)td.L0 + (int64_t)td.Ls * col_last);
     const int64_t q = (lr_blk + INIT_PARTS - 1) / INIT_PARTS;
     const int64_t l0 = part * q, l1 = min(lr_blk, l0 + q);
-    double E[NC];
+    InitState S;
+    int64_t hi[INIT_GROUPS];  // each group's longest window (uniform)
 #pragma unroll
-    for (int c = 0; c < NC; c++) E[c] = 0.0;
-    if (l0 < l1) {  // block-uniform
-        const double tau = (double)td.tau + (double)(valid ? n : col_last) * (double)td.dtau;
+    for (int i = 0; i < INIT_GROUPS; i++) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) S.E[i][c] = 0.0;
+        const int g0 = col0 + INIT_WG * i;
+        const int glast = min(g0 + INIT_WG - 1, col_last);
+        const int n = min(g0 + lane, col_last);  // lanes past the map follow the last column
+        const bool any = g0 <= col_last;
+        S.lr[i] = any ? clip((int64_t)td.L0 + (int64_t)td.Ls * n) : 0;
+        S.lo[i] = any ? clip((int64_t)td.L0 + (int64_t)td.Ls * g0) : 0;
+        hi[i] = any ? clip((int64_t)td.L0 + (int64_t)td.Ls * glast) : 0;
+        const double tau = (double)td.tau + (double)n * (double)td.dtau;
+        const double r1 = exp(-(double)td.TAtom / tau);
+        S.rho[i][0] = r1 * r1;
+        S.rho[i][1] = r1;
         const double w1 = exp(-(double)l0 * (double)td.TAtom / tau);  // rho^l0
-        double w[2] = {w1 * w1, w1};
-        // rows k0 + l, l < l1 <= room, lie below H <= N
-        const double *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
-        for (int64_t c0 = l0; c0 < l1; c0 += INIT_CHUNK) {
-            const int na = (int)min((int64_t)INIT_CHUNK, l1 - c0);
-            __syncthreads();  // the previous chunk is consumed
-            for (int i = threadIdx.x; i < na * MERGED_STRIDE; i += WG)
-                s_x[i] = X[c0 * MERGED_STRIDE + i];
-            __syncthreads();
-            // this lane's window ends at lr: a wave whose lanes are all past
-            // their windows leaves the loop (window lengths grow with n, so
-            // the early waves of a block finish first)
-            const int jn = (int)max((int64_t)0, min((int64_t)na, lr - c0));
-#pragma unroll 4
-            for (int j = 0; j < jn; j++) {
+        S.w[i][0] = w1 * w1;
+        S.w[i][1] = w1;
+    }
+    // Windows grow with the column: group i's shortest window (lo) is at
+    // least group i-1's longest (hi).  Atoms below lo[0] lie inside every
+    // window; segment I0 = [hi[I0-1], hi[I0]) needs group I0 masked and no
+    // group before it.
+    const int64_t seg_end[5] = {S.lo[0], hi[0], hi[1], hi[2], hi[3]};
+    // rows k0 + l, l < l1 <= room, lie below H
+    const double *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
+    double r[INIT_PER];
+    auto fetch = [&](int64_t c0) {
+        const int64_t nq = min((int64_t)INIT_CHUNK, l1 - c0) * MERGED_STRIDE;
 #pragma unroll
-                for (int c = 0; c < NC; c++)
-                    E[c] = fma(w[c < 3 ? 0 : 1], s_x[j * MERGED_STRIDE + c], E[c]);
-                w[0] *= ec.rho[0];
-                w[1] *= ec.rho[1];
+        for (int k = 0; k < INIT_PER; k++) {
+            const int64_t qd = (int64_t)k * INIT_WG + lane;
+            r[k] = qd < nq ? X[c0 * MERGED_STRIDE + qd] : 0.0;
+        }
+    };
+    if (l0 < l1) fetch(l0);
+    for (int64_t c0 = l0; c0 < l1; c0 += INIT_CHUNK) {
+        const int64_t c1 = min(l1, c0 + INIT_CHUNK);
+        __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+        for (int k = 0; k < INIT_PER; k++) buf[k * INIT_WG + lane] = r[k];
+        __syncthreads();
+        if (c1 < l1) fetch(c1);  // in flight while this chunk is summed
+        int64_t sa = c0;
+#pragma unroll
+        for (int sgm = 0; sgm < 5; sgm++) {
+            const int64_t sb = min(c1, max(sa, seg_end[sgm]));
+            if (sa < sb) {  // uniform
+                if (sgm == 0) init_segment<0, false>(S, buf, c0, sa, sb);
+                if (sgm == 1) init_segment<0, true>(S, buf, c0, sa, sb);
+                if (sgm == 2) init_segment<1, true>(S, buf, c0, sa, sb);
+                if (sgm == 3) init_segment<2, true>(S, buf, c0, sa, sb);
+                if (sgm == 4) init_segment<3, true>(S, buf, c0, sa, sb);
             }
+            sa = sb;
         }
     }
-    if (valid) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) expf_etab(etab, td, td.n_seg + part, c)[n] = E[c];
+    for (int i = 0; i < INIT_GROUPS; i++) {
+        const int n = col0 + INIT_WG * i + lane;
+        if (n <= col_last) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) expf_etab(etab, td, td.n_seg + part, c)[n] = S.E[i][c];
+        }
     }
 }
 
@@ -1605,7 +1686,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     ExpWalk wk = expw_init(td, merged + td.merged_off, lds_raw);
     double E[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) E[c] = valid ? expf_etab((double *)etab, td, it.rb, c)[n] : 0.0;
+    for (int c = 0; c < NC; c++) E[c] = 0.0;
+    if (valid && td.n_seg > 1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) E[c] = expf_etab((double *)etab, td, it.rb, c)[n];
+    } else if (valid && !td.init_empty) {  // one chunk: E_top is the direct sum's parts
+#pragma unroll
+        for (int pp = 0; pp < INIT_PARTS; pp++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) E[c] += expf_etab((double *)etab, td, 1 + pp, c)[n];
+    }
 
     Reducer<BTSG, 1> red;
     red.init();
@@ -1971,6 +2061,7 @@ struct tcw_plan {
     std::vector<ItemDev> items;      // grouped by mode
     std::vector<ItemDev> seg_items;  // MODE_EXP_FAST (t, chunk, column block), chunk >= 1
     std::vector<ItemDev> col_items;  // MODE_EXP_FAST (t, -, column block)
+    std::vector<ItemDev> init_items;  // exp_init waves (t, part, column block), heaviest first
     std::vector<int32_t> exp_tmpls;  // templates needing the FP64 atom table
     int64_t exp_max_N = 0;
     bool exp_init_needed = false;  // some exp template has a non-empty direct sum
@@ -1982,7 +2073,7 @@ struct tcw_plan {
     int max_lds_bytes = 0;  // rect_fast_kernel dynamic LDS
     int max_lds_expf = 0;   // exp_fast_kernel dynamic LDS
     DevBuf d_tmpl, d_segs, d_items, d_prefix, d_merged, d_ipart, d_rowpart, d_colpart,
-        d_flags, d_seg_items, d_col_items, d_etab, d_exp_tmpls;
+        d_flags, d_seg_items, d_col_items, d_etab, d_exp_tmpls, d_init_items;
     hipGraphExec_t graph = nullptr;
     hipStream_t cap_stream = nullptr;  // private stream the graph is captured on
     // exp_init_kernel runs here beside exp_seg_kernel (they are independent)
@@ -2001,6 +2092,7 @@ struct tcw_plan {
         items.clear();
         seg_items.clear();
         col_items.clear();
+        init_items.clear();
         exp_tmpls.clear();
         exp_max_N = 0;
         exp_init_needed = false;
@@ -2029,6 +2121,7 @@ struct tcw_plan {
         d_flags.release();
         d_seg_items.release();
         d_col_items.release();
+        d_init_items.release();
         d_etab.release();
         d_exp_tmpls.release();
     }
@@ -2120,6 +2213,29 @@ int effective_window(uint32_t numAtoms, uint32_t TAtom, uint32_t t0_data,
         return fail(TCW_ERR_INVALID, "map of " + std::to_string(N_t0) + " x " +
                                          std::to_string(N_tau) + " cells exceeds 2^31-1 cells.");
     return TCW_OK;
+}
+
+// Atom-column products an exp_init wave does for (template, column block,
+// part): the kernel's l range of the part, each column group counted up to its
+// longest window (the order the waves are launched in: heaviest first).
+int32_t init_work(const TmplDev &td, uint32_t cb, int part) {
+    const int64_t k0 = (int64_t)td.k00 + (int64_t)td.a * ((int64_t)td.n_seg * td.rb_rows - 1);
+    const int64_t room = (int64_t)td.H - k0;
+    auto clip = [&](int64_t L) { return std::max<int64_t>(std::min<int64_t>(L, room), 0); };
+    const int64_t col0 = (int64_t)cb * TILE_COLS;
+    const int64_t col_last = std::min<int64_t>(col0 + TILE_COLS, td.N_tau) - 1;
+    const int64_t lr_blk = clip((int64_t)td.L0 + (int64_t)td.Ls * col_last);
+    const int64_t q = (lr_blk + INIT_PARTS - 1) / INIT_PARTS;
+    const int64_t l0 = part * q, l1 = std::min<int64_t>(lr_blk, l0 + q);
+    int64_t w = 0;
+    for (int i = 0; i < INIT_GROUPS; i++) {
+        const int64_t g0 = col0 + (int64_t)INIT_WG * i;
+        if (g0 > col_last) break;
+        const int64_t glast = std::min<int64_t>(g0 + INIT_WG - 1, col_last);
+        const int64_t hi = clip((int64_t)td.L0 + (int64_t)td.Ls * glast);
+        w += std::max<int64_t>(0, std::min(l1, hi) - l0);
+    }
+    return (int32_t)std::min<int64_t>(w, 0x7fffffff);
 }
 
 // Build one template's descriptor and items.  seg range and N/t_min precomputed.
@@ -2236,7 +2352,13 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
             // the top chunk's top row lies at or beyond H: its direct sum is
             // empty, exp_scan starts from 0 and exp_init need not run
             td.init_empty = (int64_t)td.H - ((int64_t)td.k00 + (int64_t)td.a * (nseg * rb - 1)) <= 0;
-            if (!td.init_empty) p.exp_init_needed = true;
+            if (!td.init_empty) {
+                p.exp_init_needed = true;
+                for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
+                    for (int part = 0; part < INIT_PARTS; part++)
+                        p.init_items.push_back(ItemDev{(int32_t)t, part, (int32_t)cbk,
+                                                       init_work(td, cbk, part)});
+            }
             // few (template, column block) pairs, each with many chunks: a
             // wave per column scans the chunks in log depth; otherwise a
             // thread per column walks them (coalesced, no idle lanes)
@@ -2247,8 +2369,9 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
             for (int64_t sg = 1; sg < nseg; sg++)
                 for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
                     p.seg_items.push_back(ItemDev{(int32_t)t, (int32_t)sg, (int32_t)cbk, 0});
-            for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
-                p.col_items.push_back(ItemDev{(int32_t)t, 0, (int32_t)cbk, 0});
+            if (nseg > 1)  // one chunk: the map kernel adds the direct sum's parts
+                for (uint32_t cbk = 0; cbk < td.n_colblk; cbk++)
+                    p.col_items.push_back(ItemDev{(int32_t)t, 0, (int32_t)cbk, 0});
             if (expw_pass_rows(td.a, td.Ls, (int)std::min<int64_t>(TILE_COLS, N_tau)) > 0)
                 p.max_lds_expf = EXPF_LDS_BYTES;  // fixed plane strides
         }
@@ -2294,6 +2417,7 @@ void plan_buffers(tcw_plan &p, F f) {
     f(p.d_flags, sizeof(int) * std::max<int64_t>(p.T, 1));
     f(p.d_seg_items, sizeof(ItemDev) * std::max<size_t>(p.seg_items.size(), 1));
     f(p.d_col_items, sizeof(ItemDev) * std::max<size_t>(p.col_items.size(), 1));
+    f(p.d_init_items, sizeof(ItemDev) * std::max<size_t>(p.init_items.size(), 1));
     f(p.d_etab, sizeof(double) * std::max<int64_t>(p.etab_doubles, 1));
     f(p.d_exp_tmpls, sizeof(int32_t) * std::max<size_t>(p.exp_tmpls.size(), 1));
 }
@@ -2304,6 +2428,10 @@ void plan_buffers(tcw_plan &p, F f) {
 // when given, else synchronously).
 int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
                 hipStream_t st, bool async) {
+    // exp_init waves heaviest first (work estimate in .pad): the last waves
+    // to start are the short ones
+    std::stable_sort(p.init_items.begin(), p.init_items.end(),
+                     [](const ItemDev &x, const ItemDev &y) { return x.pad > y.pad; });
     p.items.clear();
     for (int md = 0; md < MODE_COUNT; md++) {
         p.mode_begin[md] = (int64_t)p.items.size();
@@ -2344,6 +2472,7 @@ int plan_finish(tcw_plan &p, std::vector<ItemDev> (&items_by_mode)[MODE_COUNT],
     TCW_HIP_TRY(up(p.d_items.p, p.items.data(), sizeof(ItemDev) * p.items.size()));
     TCW_HIP_TRY(up(p.d_seg_items.p, p.seg_items.data(), sizeof(ItemDev) * p.seg_items.size()));
     TCW_HIP_TRY(up(p.d_col_items.p, p.col_items.data(), sizeof(ItemDev) * p.col_items.size()));
+    TCW_HIP_TRY(up(p.d_init_items.p, p.init_items.data(), sizeof(ItemDev) * p.init_items.size()));
     TCW_HIP_TRY(up(p.d_exp_tmpls.p, p.exp_tmpls.data(), sizeof(int32_t) * p.exp_tmpls.size()));
     return TCW_OK;
 }
@@ -2465,17 +2594,18 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     }
     // exp_init and exp_seg read the atom table and write different slots of the
     // E table: with enough blocks for both, exp_init runs on the side stream
-    const int64_t init_blocks = p.exp_init_needed ? (int64_t)p.col_items.size() * INIT_PARTS : 0;
+    // exp_init runs one wave per item; blocks of 4 waves as the unit here
+    const int64_t init_blocks = p.exp_init_needed ? (int64_t)p.init_items.size() / INIT_GROUPS : 0;
     const bool fork = p.side && init_blocks > 0 && !p.seg_items.empty() &&
                       init_blocks + (int64_t)p.seg_items.size() >= EXP_FORK_MIN_BLOCKS;
     if (fork) {
         TCW_HIP_TRY(hipEventRecord(p.ev_fork, st));
         TCW_HIP_TRY(hipStreamWaitEvent(p.side, p.ev_fork, 0));
     }
-    if (init_blocks > 0)
-        hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)init_blocks), dim3(WG), 0,
+    if (p.exp_init_needed)
+        hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)p.init_items.size()), dim3(INIT_WG), 0,
                            fork ? p.side : st, (const TmplDev *)p.d_tmpl.p,
-                           (const ItemDev *)p.d_col_items.p, (const double *)p.d_merged.p,
+                           (const ItemDev *)p.d_init_items.p, (const double *)p.d_merged.p,
                            (double *)p.d_etab.p);
     if (!p.seg_items.empty())
         hipLaunchKernelGGL(exp_seg_kernel, dim3((unsigned)p.seg_items.size()), dim3(WG),
@@ -2636,7 +2766,7 @@ int64_t tcw_plan_scratch_bytes(const tcw_plan *p) {
     return (int64_t)(p->d_tmpl.cap + p->d_segs.cap + p->d_items.cap + p->d_prefix.cap +
                      p->d_merged.cap + p->d_ipart.cap + p->d_rowpart.cap + p->d_colpart.cap +
                      p->d_flags.cap + p->d_seg_items.cap + p->d_col_items.cap +
-                     p->d_etab.cap);
+                     p->d_etab.cap + p->d_init_items.cap);
 }
 
 int tcw_plan_items(const tcw_plan *p, int64_t *items) {
