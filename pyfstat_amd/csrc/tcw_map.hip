@@ -314,6 +314,47 @@ constexpr int SX_CHUNK = 1024;  // atoms staged per pass
 // consecutive lanes on consecutive doubles
 constexpr int SX_ROW = MERGED_STRIDE + 1;
 
+// For sorted ts[lo0, hi0): the first index with ts >= key0 into s_r[0][0]
+// and with ts >= key1 into s_r[1][0] (hi0 if none).  The two halves of a
+// 256-thread block search one key each, 128-way: each round samples 128
+// evenly spaced timestamps of the remaining range (one load per thread, all
+// in flight together) and keeps the gap where the key falls, so a range of
+// n atoms takes about log_128(n) rounds of loads instead of log_2(n)
+// dependent ones.  Every thread of the block calls it; the results are valid
+// on return.
+__device__ __forceinline__ void block_lower_bound2(const uint32_t *__restrict__ ts, int64_t lo0,
+                                                   int64_t hi0, uint64_t key0, uint64_t key1,
+                                                   int64_t (&s_r)[2][2], int (&s_c)[4]) {
+    const int tid = threadIdx.x, half = tid >> 7, t = tid & 127, wave = tid >> 6;
+    const uint64_t key = half ? key1 : key0;
+    if (tid < 2) {
+        s_r[tid][0] = lo0;
+        s_r[tid][1] = hi0;
+    }
+    __syncthreads();
+    // the answer lies in [lo, hi]; n = hi - lo timestamps are left to test
+    while (s_r[0][1] > s_r[0][0] || s_r[1][1] > s_r[1][0]) {  // block-uniform
+        const int64_t lo = s_r[half][0], hi = s_r[half][1];
+        const int64_t n = hi - lo;
+        const int64_t sd = n <= 128 ? 1 : (n + 127) / 128;
+        const bool pred = n > 0 && t * sd < n && (uint64_t)ts[lo + t * sd] < key;
+        const uint64_t b = __ballot(pred);
+        if ((tid & 63) == 0) s_c[wave] = __popcll(b);
+        __syncthreads();
+        if (t == 0 && n > 0) {
+            // c samples lie below the key: the answer is in (sample c-1, sample c]
+            const int c = s_c[2 * half] + s_c[2 * half + 1];
+            if (c == 0) {
+                s_r[half][1] = lo;
+            } else {
+                s_r[half][0] = lo + (int64_t)(c - 1) * sd + 1;
+                s_r[half][1] = min(lo + (int64_t)c * sd, hi);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int SXB>
 __global__ __launch_bounds__(256) void stage_exp_kernel(
     const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
@@ -325,7 +366,8 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
     __shared__ float s_raw[SX_RAW];
     uint32_t *s_ts = reinterpret_cast<uint32_t *>(s_raw);
     float(*s_x)[SX_CHUNK] = reinterpret_cast<float(*)[SX_CHUNK]>(s_raw + SX_CHUNK);
-    __shared__ int64_t s_rng[2];
+    __shared__ int64_t s_rng[2][2];
+    __shared__ int s_cnt[4];
     const int tid = threadIdx.x;
     for (int64_t e = blockIdx.y; e < n_exp; e += gridDim.y) {
         const TmplDev td = tds[exp_tmpls[e]];
@@ -346,20 +388,9 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
             for (int c = 0; c < NC; c++) v[j][c] = 0.0f;
         for (int64_t X = td.seg_begin; X < td.seg_end; X++) {
             const SegDev sg = segs[X];
-            if (tid < 2) {  // first atom with ts >= tb_lo, resp. >= tb_hi
-                const uint64_t key = tid == 0 ? tb_lo : tb_hi;
-                int64_t lo = sg.begin, hi = sg.end;
-                while (lo < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if ((uint64_t)ts[mid] < key)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                s_rng[tid] = lo;
-            }
-            __syncthreads();
-            const int64_t ka = s_rng[0], kb = s_rng[1];
+            // first atom with ts >= tb_lo, resp. >= tb_hi
+            block_lower_bound2(ts, sg.begin, sg.end, tb_lo, tb_hi, s_rng, s_cnt);
+            const int64_t ka = s_rng[0][0], kb = s_rng[1][0];
             __syncthreads();  // s_rng is rewritten by the next detector
             for (int64_t c0 = ka; c0 < kb; c0 += SX_CHUNK) {
                 const int na = (int)min((int64_t)SX_CHUNK, kb - c0);
