@@ -4,7 +4,7 @@
 //   1. stage_kernel    binned detector merge (lalpulsar mergeMultiFstatAtomsBinned,
 //                      called at pyfstat/tcw_fstat_map_funcs.py:706) fused with an
 //                      FP64 prefix scan of the float32 merged bins (rect windows);
-//                      stage_exp_kernel: the same merge into an FP64 atom table
+//                      stage_exp_kernel: the same merge into a float32 atom table
 //                      (exp windows);
 //   2. map kernels     window sums -> cond-guarded F per (t0, tau) cell, with the
 //                      max/argmax, logsumexp and row/column posterior sums fused
@@ -50,7 +50,7 @@ namespace {
 constexpr int NC = TCW_NCOL;
 constexpr int WG = 256;           // threads per workgroup (4 wave64)
 constexpr int TILE_COLS = 256;    // tau columns per work item
-constexpr int MERGED_STRIDE = 8;  // doubles per atom in the FP64 atom table
+constexpr int MERGED_STRIDE = 8;  // floats per atom in the atom table (7 columns + pad)
 constexpr double COND_MAX = 1e4;  // rect.cu:104
 // cond < K  <=>  T^2 < Q*D  with T = A+B, D = AB-C^2 > 0, Q = (K+1)^2/K, because
 // cond = (T+s)/(T-s), s^2 = T^2-4D, and (1+cond)^2/cond = T^2/D grows with cond.
@@ -70,7 +70,7 @@ constexpr int EXP_SMALL_BATCH_ROWS = 64;    // exp rows when untuned and under 2
 
 struct TmplDev {
     int64_t seg_begin, seg_end;  // detector segments of this template
-    int64_t merged_off;          // doubles, FP64 atom table [N][8]   (exp modes)
+    int64_t merged_off;          // floats, atom table [bins][8]      (exp modes)
     int64_t prefix_off;          // doubles, prefix entries [N+1][7]  (rect modes)
     int64_t F_off;               // floats, map [N_t0][N_tau]
     int64_t item_begin;          // first work item of this template
@@ -111,7 +111,7 @@ struct ItemPart {
 };
 
 // ----------------------------------------------------------------------------
-// stage: binned merge + FP64 prefix (rect) / FP64 atom table (exp)
+// stage: binned merge + FP64 prefix (rect) / float32 atom table (exp)
 // ----------------------------------------------------------------------------
 
 constexpr int SR_WG = 256;                        // 4 wave64
@@ -292,16 +292,17 @@ __global__ __launch_bounds__(SR_WG) void stage_kernel(
     if (tid < NC) P[(int64_t)N * NC + tid] = s_carry[tid];
 }
 
-// Bins of a template's FP64 atom table: the recurrence path reads none at or
+// Bins of a template's atom table: the recurrence path reads none at or
 // past H (the first atom no window reaches), the direct path any of the N.
 __host__ __device__ __forceinline__ uint32_t exp_table_bins(const TmplDev &td) {
     return td.mode == MODE_EXP_FAST ? (uint32_t)td.H : td.N;
 }
 
-// Exponential-window templates: the merged atoms as the FP64 table
+// Exponential-window templates: the merged atoms as the float32 table
 // [exp_table_bins][8] (7 columns + padding), the same float32 detector-order
-// sums as stage_kernel.  A workgroup owns 256 * SXB consecutive bins of one template:
-// per detector, two binary searches bound the block's atoms, which are copied
+// sums as stage_kernel (the kernels that read it widen them to FP64, exactly).
+// A workgroup owns 256 * SXB consecutive bins of one template: per detector,
+// a block-wide search bounds the block's atoms, which are copied
 // into LDS in chunks with coalesced loads; each thread then sums its SXB bins
 // from LDS (atoms in order, detectors in order).  Grid: (bin blocks, exp
 // templates), y-strided.  SXB = 4 (1024-bin blocks) for batches, 1 (256-bin
@@ -310,8 +311,8 @@ __host__ __device__ __forceinline__ uint32_t exp_table_bins(const TmplDev &td) {
 constexpr int SX_NARROW_BELOW = 256;
 constexpr int SX_CHUNK = 1024;  // atoms staged per pass
 // the finished rows are restaged in LDS (row stride SX_ROW floats, odd so the
-// lanes' writes spread over the banks) and written to the FP64 table with
-// consecutive lanes on consecutive doubles
+// lanes' writes spread over the banks) and written to the table with
+// consecutive lanes on consecutive floats
 constexpr int SX_ROW = MERGED_STRIDE + 1;
 
 // For sorted ts[lo0, hi0): the first index with ts >= key0 into s_r[0][0]
@@ -359,7 +360,7 @@ template <int SXB>
 __global__ __launch_bounds__(256) void stage_exp_kernel(
     const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ ts,
-    const float *__restrict__ cols, int64_t cstride, double *__restrict__ merged) {
+    const float *__restrict__ cols, int64_t cstride, float *__restrict__ merged) {
     // atom staging during the merge, the finished rows afterwards
     constexpr int SX_RAW = (NC + 1) * SX_CHUNK > 256 * SXB * SX_ROW ? (NC + 1) * SX_CHUNK
                                                                      : 256 * SXB * SX_ROW;
@@ -432,10 +433,10 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
             r[NC] = 0.0f;
         }
         __syncthreads();
-        double *base = merged + td.merged_off + (int64_t)B0 * MERGED_STRIDE;
+        float *base = merged + td.merged_off + (int64_t)B0 * MERGED_STRIDE;
         const int nq = (int)(B1 - B0) * MERGED_STRIDE;
         for (int q = tid; q < nq; q += 256)
-            base[q] = (double)s_raw[(q / MERGED_STRIDE) * SX_ROW + q % MERGED_STRIDE];
+            base[q] = s_raw[(q / MERGED_STRIDE) * SX_ROW + q % MERGED_STRIDE];
         __syncthreads();  // s_raw is staged again for the next template
     }
 }
@@ -603,14 +604,14 @@ __device__ __forceinline__ float rows_sum<4>(const float (&v)[4], int lane, int 
 //    wave's running maximum (accumulators rescaled when it grows) -- one FMA
 //    and one exp2 per cell; each sum is relative to its reference in log2
 //    units (ML), i.e. to exp(F - ML/LOG2E).  Column sums stay in registers
-//    across sub-tiles, row sums are wave-reduced and parked in LDS with ML.
+//    across sub-tiles (their sum is the item total), row sums are
+//    wave-reduced and parked in LDS with ML.
 template <bool BTSG, int NCOLS = 4>
 struct Reducer {
     float best;
     int bidx;
     float Mw;
     float ML;
-    float Stot;
     float Scol[NCOLS];
 
     __device__ void init() {
@@ -618,9 +619,16 @@ struct Reducer {
         bidx = 0x7fffffff;
         Mw = -INFINITY;
         ML = -INFINITY;
-        Stot = 0.0f;
 #pragma unroll
         for (int c = 0; c < NCOLS; c++) Scol[c] = 0.0f;
+    }
+
+    // the lane's sum over all its cells (relative to ML)
+    __device__ float total() const {
+        float s = Scol[0];
+#pragma unroll
+        for (int c = 1; c < NCOLS; c++) s += Scol[c];
+        return s;
     }
 
     // F[r][c]: cell values (-inf when invalid); row index m0 + r; colidx[c]
@@ -659,7 +667,6 @@ struct Reducer {
             const float wm = wave_max(lm);
             const float wmL = wm * LOG2E;
             const float sc = exp2_diff(ML, wmL);
-            Stot *= sc;
 #pragma unroll
             for (int c = 0; c < NCOLS; c++) Scol[c] *= sc;
             Mw = wm;
@@ -676,7 +683,6 @@ struct Reducer {
                 sr[r] = c == 0 ? e : sr[r] + e;
                 Scol[c] += e;
             }
-            Stot += sr[r];
         }
         int row;
         const float srw = rows_sum<R>(sr, lane, row);
@@ -704,7 +710,7 @@ __device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const
 
     wave_argmax(red.best, red.bidx);
     float stot = 0.0f;
-    if (BTSG) stot = wave_sum(red.Stot);
+    if (BTSG) stot = wave_sum(red.total());
     if (lane == 0) {
         sh_best[wave] = red.best;
         sh_bidx[wave] = red.bidx;
@@ -996,7 +1002,7 @@ __global__ __launch_bounds__(WG) void rect_generic_kernel(
 template <bool BTSG, bool STRICT>
 __global__ __launch_bounds__(WG) void exp_direct_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
-    const double *__restrict__ merged, float *__restrict__ Fout,
+    const float *__restrict__ merged, float *__restrict__ Fout,
     ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
     float *__restrict__ colpart, int *__restrict__ flags) {
     __shared__ float sh_rowM[MAX_RB];
@@ -1008,7 +1014,7 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
     const int row0 = it.rb * RB;
     const int col0 = it.cb * TILE_COLS;
     const int N_t0 = (int)td.N_t0, N_tau = (int)td.N_tau;
-    const double *__restrict__ X = merged + td.merged_off;
+    const float *__restrict__ X = merged + td.merged_off;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     Reducer<BTSG> red;
@@ -1055,7 +1061,7 @@ __global__ __launch_bounds__(WG) void exp_direct_kernel(
             for (int64_t i = lo; i <= hi_w; i++) {
                 const double wi = (i <= hi) ? w : 0.0;
                 const double w2 = wi * wi;
-                const double *xi = X + i * MERGED_STRIDE;
+                const float *xi = X + i * MERGED_STRIDE;
                 acc[0] = fma(xi[0], w2, acc[0]);
                 acc[1] = fma(xi[1], w2, acc[1]);
                 acc[2] = fma(xi[2], w2, acc[2]);
@@ -1167,7 +1173,7 @@ __host__ __device__ __forceinline__ int expw_pass_rows(int a, int Ls, int cols) 
 }
 
 struct ExpWalk {
-    const double *X;  // the template's FP64 atom table
+    const float *X;   // the template's atom table (float32 merged sums)
     double *tl;       // LDS tail planes [c][EXPW_TAIL_CAP]
     double *hd;       // LDS head atoms [j][MERGED_STRIDE]
     int64_t head_lo;  // atom in head slot 0
@@ -1175,7 +1181,7 @@ struct ExpWalk {
     int H;
 };
 
-__device__ __forceinline__ ExpWalk expw_init(const TmplDev &td, const double *X, double *lds) {
+__device__ __forceinline__ ExpWalk expw_init(const TmplDev &td, const float *X, double *lds) {
     ExpWalk wk;
     wk.X = X;
     wk.tl = lds;
@@ -1200,8 +1206,8 @@ static_assert(EXPW_TAIL_CAP % 32 == 0 && WG == 256, "staging decomposition");
 struct ExpPass {
     int64_t head_lo, tail_lo;
     int nhead, ntail;
-    double h[EXPW_HEAD_PER];
-    double t[EXPW_TAIL_PER];
+    float h[EXPW_HEAD_PER];  // exact: the table holds float32 sums
+    float t[EXPW_TAIL_PER];
 };
 
 __device__ __forceinline__ void expw_fetch(const ExpWalk &wk, const TmplDev &td, int st, int sb,
@@ -1215,7 +1221,7 @@ __device__ __forceinline__ void expw_fetch(const ExpWalk &wk, const TmplDev &td,
     for (int k = 0; k < EXPW_HEAD_PER; k++) {
         const int j = j0 + 32 * k;
         const int64_t atom = ps.head_lo + j;
-        ps.h[k] = j < ps.nhead && atom >= 0 && atom < wk.H ? wk.X[atom * MERGED_STRIDE + c] : 0.0;
+        ps.h[k] = j < ps.nhead && atom >= 0 && atom < wk.H ? wk.X[atom * MERGED_STRIDE + c] : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < EXPW_TAIL_PER; k++) {
@@ -1223,7 +1229,7 @@ __device__ __forceinline__ void expw_fetch(const ExpWalk &wk, const TmplDev &td,
         const int64_t atom = ps.tail_lo + j;
         ps.t[k] = c < NC && j < ps.ntail && atom >= 0 && atom < wk.H
                       ? wk.X[atom * MERGED_STRIDE + c]
-                      : 0.0;
+                      : 0.0f;
     }
 }
 
@@ -1235,12 +1241,12 @@ __device__ __forceinline__ void expw_put(ExpWalk &wk, const ExpPass &ps) {
 #pragma unroll
     for (int k = 0; k < EXPW_HEAD_PER; k++) {
         const int j = j0 + 32 * k;
-        if (j < ps.nhead) wk.hd[j * MERGED_STRIDE + c] = ps.h[k];
+        if (j < ps.nhead) wk.hd[j * MERGED_STRIDE + c] = (double)ps.h[k];
     }
 #pragma unroll
     for (int k = 0; k < EXPW_TAIL_PER; k++) {
         const int j = j0 + 32 * k;
-        if (c < NC && j < ps.ntail) wk.tl[c * EXPW_TAIL_CAP + j] = ps.t[k];
+        if (c < NC && j < ps.ntail) wk.tl[c * EXPW_TAIL_CAP + j] = (double)ps.t[k];
     }
 }
 
@@ -1273,13 +1279,13 @@ __device__ __forceinline__ void expw_step_table(const ExpWalk &wk, const ExpCol 
     for (int l = 1; l <= a; l++) {
         const int64_t kk = k - l;     // entering atom (wave-uniform)
         const int64_t kt = kk + ec.L;  // leaving atom
-        const double *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
-        const double *pt = wk.X + min(kt, (int64_t)wk.H - 1) * MERGED_STRIDE;
+        const float *ph = wk.X + min(kk, (int64_t)wk.H - 1) * MERGED_STRIDE;
+        const float *pt = wk.X + min(kt, (int64_t)wk.H - 1) * MERGED_STRIDE;
         double xh[NC], xt[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            xh[c] = kk < wk.H ? ph[c] : 0.0;
-            xt[c] = kt < wk.H ? pt[c] : 0.0;
+            xh[c] = kk < wk.H ? (double)ph[c] : 0.0;
+            xt[c] = kt < wk.H ? (double)pt[c] : 0.0;
         }
         expw_step(ec, xh, xt, E);
     }
@@ -1294,7 +1300,7 @@ __device__ __forceinline__ double *expf_etab(double *etab, const TmplDev &td, in
 // top row of chunk s-1) started from zero.  Work items (t, s, cb).
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void exp_seg_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items,
-    const double *__restrict__ merged, double *__restrict__ etab, int lds_ok) {
+    const float *__restrict__ merged, double *__restrict__ etab, int lds_ok) {
     extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
     const ItemDev it = items[block_slot()];
     const TmplDev td = tds[it.t];
@@ -1413,7 +1419,7 @@ __device__ __forceinline__ void init_segment(InitState &S, const double *buf, in
 
 __global__ __launch_bounds__(INIT_WG) void exp_init_kernel(const TmplDev *__restrict__ tds,
                                                            const ItemDev *__restrict__ items,
-                                                           const double *__restrict__ merged,
+                                                           const float *__restrict__ merged,
                                                            double *__restrict__ etab) {
     __shared__ double buf[INIT_CHUNK * MERGED_STRIDE];
     const ItemDev it = items[blockIdx.x];  // (t, part, column block)
@@ -1456,14 +1462,14 @@ __global__ __launch_bounds__(INIT_WG) void exp_init_kernel(const TmplDev *__rest
     // group before it.
     const int64_t seg_end[5] = {S.lo[0], hi[0], hi[1], hi[2], hi[3]};
     // rows k0 + l, l < l1 <= room, lie below H
-    const double *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
-    double r[INIT_PER];
+    const float *__restrict__ X = merged + td.merged_off + k0 * MERGED_STRIDE;
+    float r[INIT_PER];  // exact: the table holds float32 sums
     auto fetch = [&](int64_t c0) {
         const int64_t nq = min((int64_t)INIT_CHUNK, l1 - c0) * MERGED_STRIDE;
 #pragma unroll
         for (int k = 0; k < INIT_PER; k++) {
             const int64_t qd = (int64_t)k * INIT_WG + lane;
-            r[k] = qd < nq ? X[c0 * MERGED_STRIDE + qd] : 0.0;
+            r[k] = qd < nq ? X[c0 * MERGED_STRIDE + qd] : 0.0f;
         }
     };
     if (l0 < l1) fetch(l0);
@@ -1471,7 +1477,7 @@ __global__ __launch_bounds__(INIT_WG) void exp_init_kernel(const TmplDev *__rest
         const int64_t c1 = min(l1, c0 + INIT_CHUNK);
         __syncthreads();  // the previous chunk is consumed
 #pragma unroll
-        for (int k = 0; k < INIT_PER; k++) buf[k * INIT_WG + lane] = r[k];
+        for (int k = 0; k < INIT_PER; k++) buf[k * INIT_WG + lane] = (double)r[k];
         __syncthreads();
         if (c1 < l1) fetch(c1);  // in flight while this chunk is summed
         int64_t sa = c0;
@@ -1631,8 +1637,10 @@ __device__ __forceinline__ void expw_rows(const TmplDev &td, const ExpWalk &wk, 
         BUF_RSRC_WORD3);
     uint32_t vo = valid ? (uint32_t)(st - sb) * rowb + (uint32_t)n * (uint32_t)sizeof(float)
                         : 0x80000000u;  // row st
-    int jh = st - sb + 1;  // LDS path: head slot of row st's step
-    int jt = jh + jt_off;  // and tail slot
+    // LDS path: head and tail slots of the step at the group's first row;
+    // the group's rows address slots below them by constants
+    int jh = st - sb + 1;
+    int jt = jh + jt_off;
     double xh[NC], xt[NC];
     if (LDS) expw_load(wk, jh, jt, xh, xt);
     for (int g = st; g >= sb; g -= 4) {
@@ -1642,7 +1650,7 @@ __device__ __forceinline__ void expw_rows(const TmplDev &td, const ExpWalk &wk, 
         for (int j = 0; j < 4; j++) {
             const int m = g - j;
             double nh[NC], nt[NC];
-            if (LDS) expw_load(wk, jh - 1, jt - 1, nh, nt);  // the next row's atoms
+            if (LDS) expw_load(wk, jh - 1 - j, jt - 1 - j, nh, nt);  // the next row's atoms
             // every lane evaluates F (no branch around it, so the compiler
             // can overlap the rows' evaluations); stores and masks follow
             F4[3 - j][0] = cell_F(E[0], E[1], E[2], E[3], E[4], E[5], E[6]);
@@ -1660,11 +1668,11 @@ __device__ __forceinline__ void expw_rows(const TmplDev &td, const ExpWalk &wk, 
                     xh[c] = nh[c];
                     xt[c] = nt[c];
                 }
-                jh--;
-                jt--;
             }
             k -= td.a;
         }
+        jh -= 4;
+        jt -= 4;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int m = g - 3 + r;
@@ -1687,7 +1695,7 @@ template <bool BTSG, bool STRICT>
 // Two workgroups share a CU (LDS): registers for two waves per SIMD.
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void exp_fast_kernel(
     const TmplDev *__restrict__ tds, const ItemDev *__restrict__ items, int64_t item0,
-    const double *__restrict__ merged, const double *__restrict__ etab,
+    const float *__restrict__ merged, const double *__restrict__ etab,
     float *__restrict__ Fout, ItemPart *__restrict__ ipart, float *__restrict__ rowpart,
     float *__restrict__ colpart, int *__restrict__ flags, int lds_ok) {
     extern __shared__ double lds_raw[];  // one dynamic LDS symbol per file
@@ -1772,7 +1780,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     // combine: each column belongs to one lane; each row to all 4 waves
     wave_argmax(red.best, red.bidx);
     float stot = 0.0f;
-    if (BTSG) stot = wave_sum(red.Stot);
+    if (BTSG) stot = wave_sum(red.total());
     if (lane == 0) {
         sh_best[wave] = red.best;
         sh_bidx[wave] = red.bidx;
@@ -2093,12 +2101,12 @@ struct tcw_plan {
     std::vector<ItemDev> seg_items;  // MODE_EXP_FAST (t, chunk, column block), chunk >= 1
     std::vector<ItemDev> col_items;  // MODE_EXP_FAST (t, -, column block)
     std::vector<ItemDev> init_items;  // exp_init waves (t, part, column block), heaviest first
-    std::vector<int32_t> exp_tmpls;  // templates needing the FP64 atom table
+    std::vector<int32_t> exp_tmpls;  // templates needing the atom table
     int64_t exp_max_N = 0;
     bool exp_init_needed = false;  // some exp template has a non-empty direct sum
     bool exp_scan_wave = false;    // the wave-per-column chunk scan (few, long columns)
     int64_t mode_begin[MODE_COUNT + 1] = {0, 0, 0, 0, 0};
-    int64_t prefix_doubles = 0, merged_doubles = 0, rowpart_n = 0, colpart_n = 0;
+    int64_t prefix_doubles = 0, merged_floats = 0, rowpart_n = 0, colpart_n = 0;
     int64_t etab_doubles = 0;
     int64_t F_total = 0;
     int max_lds_bytes = 0;  // rect_fast_kernel dynamic LDS
@@ -2128,7 +2136,7 @@ struct tcw_plan {
         exp_max_N = 0;
         exp_init_needed = false;
         exp_scan_wave = false;
-        prefix_doubles = merged_doubles = rowpart_n = colpart_n = F_total = 0;
+        prefix_doubles = merged_floats = rowpart_n = colpart_n = F_total = 0;
         etab_doubles = 0;
         max_lds_bytes = 0;
         max_lds_expf = 0;
@@ -2407,8 +2415,8 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
                 p.max_lds_expf = EXPF_LDS_BYTES;  // fixed plane strides
         }
         const int64_t nbins = exp_table_bins(td);
-        td.merged_off = p.merged_doubles;
-        p.merged_doubles += nbins * MERGED_STRIDE;
+        td.merged_off = p.merged_floats;
+        p.merged_floats += nbins * MERGED_STRIDE;
         p.exp_tmpls.push_back((int32_t)t);
         p.exp_max_N = std::max<int64_t>(p.exp_max_N, nbins);
     } else {
@@ -2441,7 +2449,7 @@ void plan_buffers(tcw_plan &p, F f) {
     f(p.d_segs, sizeof(SegDev) * std::max<size_t>(p.segs.size(), 1));
     f(p.d_items, sizeof(ItemDev) * std::max<size_t>(p.items.size(), 1));
     f(p.d_prefix, sizeof(double) * std::max<int64_t>(p.prefix_doubles, 1));
-    f(p.d_merged, sizeof(double) * std::max<int64_t>(p.merged_doubles, 1));
+    f(p.d_merged, sizeof(float) * std::max<int64_t>(p.merged_floats, 1));
     f(p.d_ipart, sizeof(ItemPart) * std::max<size_t>(p.items.size(), 1));
     f(p.d_rowpart, sizeof(float) * std::max<int64_t>(p.rowpart_n, 1));
     f(p.d_colpart, sizeof(float) * std::max<int64_t>(p.colpart_n, 1));
@@ -2569,7 +2577,7 @@ void launch_maps(tcw_plan &p, float *F, hipStream_t st) {
     auto *tds = (const TmplDev *)p.d_tmpl.p;
     auto *items = (const ItemDev *)p.d_items.p;
     auto *prefix = (const double *)p.d_prefix.p;
-    auto *merged = (const double *)p.d_merged.p;
+    auto *merged = (const float *)p.d_merged.p;
     auto *ipart = (ItemPart *)p.d_ipart.p;
     auto *rowpart = (float *)p.d_rowpart.p;
     auto *colpart = (float *)p.d_colpart.p;
@@ -2616,12 +2624,12 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
             hipLaunchKernelGGL(stage_exp_kernel<1>, dim3((unsigned)((p.exp_max_N + 255) / 256), gy),
                                dim3(256), 0, st, (const TmplDev *)p.d_tmpl.p,
                                (const int32_t *)p.d_exp_tmpls.p, n_exp, (const SegDev *)p.d_segs.p,
-                               ts, cols, cstride, (double *)p.d_merged.p);
+                               ts, cols, cstride, (float *)p.d_merged.p);
         else if (n_exp > 0)
             hipLaunchKernelGGL(stage_exp_kernel<4>, dim3((unsigned)wide_blocks, gy), dim3(256), 0,
                                st, (const TmplDev *)p.d_tmpl.p, (const int32_t *)p.d_exp_tmpls.p,
                                n_exp, (const SegDev *)p.d_segs.p, ts, cols, cstride,
-                               (double *)p.d_merged.p);
+                               (float *)p.d_merged.p);
     }
     // exp_init and exp_seg read the atom table and write different slots of the
     // E table: with enough blocks for both, exp_init runs on the side stream
@@ -2636,12 +2644,12 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     if (p.exp_init_needed)
         hipLaunchKernelGGL(exp_init_kernel, dim3((unsigned)p.init_items.size()), dim3(INIT_WG), 0,
                            fork ? p.side : st, (const TmplDev *)p.d_tmpl.p,
-                           (const ItemDev *)p.d_init_items.p, (const double *)p.d_merged.p,
+                           (const ItemDev *)p.d_init_items.p, (const float *)p.d_merged.p,
                            (double *)p.d_etab.p);
     if (!p.seg_items.empty())
         hipLaunchKernelGGL(exp_seg_kernel, dim3((unsigned)p.seg_items.size()), dim3(WG),
                            p.max_lds_expf, st, (const TmplDev *)p.d_tmpl.p,
-                           (const ItemDev *)p.d_seg_items.p, (const double *)p.d_merged.p,
+                           (const ItemDev *)p.d_seg_items.p, (const float *)p.d_merged.p,
                            (double *)p.d_etab.p, p.max_lds_expf > 0 ? 1 : 0);
     if (fork) {
         TCW_HIP_TRY(hipEventRecord(p.ev_join, p.side));
