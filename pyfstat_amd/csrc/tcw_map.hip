@@ -360,7 +360,8 @@ template <int SXB>
 __global__ __launch_bounds__(256) void stage_exp_kernel(
     const TmplDev *__restrict__ tds, const int32_t *__restrict__ exp_tmpls, int64_t n_exp,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ ts,
-    const float *__restrict__ cols, int64_t cstride, float *__restrict__ merged) {
+    const float *__restrict__ cols, int64_t cstride, float *__restrict__ merged,
+    int *__restrict__ flags) {
     // atom staging during the merge, the finished rows afterwards
     constexpr int SX_RAW = (NC + 1) * SX_CHUNK > 256 * SXB * SX_ROW ? (NC + 1) * SX_CHUNK
                                                                      : 256 * SXB * SX_ROW;
@@ -371,6 +372,8 @@ __global__ __launch_bounds__(256) void stage_exp_kernel(
     __shared__ int s_cnt[4];
     const int tid = threadIdx.x;
     for (int64_t e = blockIdx.y; e < n_exp; e += gridDim.y) {
+        // before any map kernel: clear the template's single-atom flag
+        if (blockIdx.x == 0 && tid == 0) flags[exp_tmpls[e]] = 0;
         const TmplDev td = tds[exp_tmpls[e]];
         constexpr int SX_BINS = SXB;
         constexpr int SX_BINS_BLOCK = 256 * SXB;
@@ -1826,23 +1829,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 // ----------------------------------------------------------------------------
 
 template <int NT>
-__device__ void block_argmax(float &bv, int &bi) {
-    __shared__ float sv[NT / 64];
-    __shared__ int si[NT / 64];
-    wave_argmax(bv, bi);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        sv[wave] = bv;
-        si[wave] = bi;
-    }
-    __syncthreads();
-    bv = sv[0];
-    bi = si[0];
-    for (int w = 1; w < NT / 64; w++) argmax_merge(bv, bi, sv[w], si[w]);
-    __syncthreads();
-}
-
-template <int NT>
 __device__ double block_sum_d(double v) {
     __shared__ double sd[NT / 64];
 #pragma unroll
@@ -1856,41 +1842,77 @@ __device__ double block_sum_d(double v) {
     return s;
 }
 
-// argmax over doubles with first-index tie break, block-wide
+constexpr int FIN_SCALE_CAP = 4096;        // per-item scale factors kept in LDS
+constexpr int64_t FIN_WIDE_MAX_T = 256;  // batches smaller than this finalize with 1024 threads
+
+// Block-wide: argmax (first index on ties) of (bv, bi) and maximum of mm.
 template <int NT>
-__device__ void block_argmax_d(double &bv, int &bi) {
-    __shared__ double sv[NT / 64];
+__device__ void block_argmax_max(float &bv, int &bi, float &mm) {
+    __shared__ float sv[NT / 64], sm[NT / 64];
     __shared__ int si[NT / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        double ov = __shfl_xor(bv, o);
-        int oi = __shfl_xor(bi, o);
-        if (ov > bv || (ov == bv && oi < bi)) {
-            bv = ov;
-            bi = oi;
-        }
-    }
+    wave_argmax(bv, bi);
+    mm = wave_max(mm);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
         sv[wave] = bv;
         si[wave] = bi;
+        sm[wave] = mm;
     }
     __syncthreads();
     bv = sv[0];
     bi = si[0];
-    for (int w = 1; w < NT / 64; w++)
-        if (sv[w] > bv || (sv[w] == bv && si[w] < bi)) {
-            bv = sv[w];
-            bi = si[w];
-        }
+    mm = sm[0];
+    for (int w = 1; w < NT / 64; w++) {
+        argmax_merge(bv, bi, sv[w], si[w]);
+        mm = fmaxf(mm, sm[w]);
+    }
     __syncthreads();
 }
 
-constexpr int FIN_SCALE_CAP = 4096;
-constexpr int64_t FIN_WIDE_MAX_T = 256;  // batches smaller than this finalize with 1024 threads  // per-item scale factors kept in LDS
+// Block-wide argmax over doubles (first index on ties) of two pairs at once.
+template <int NT>
+__device__ void block_argmax2_d(double &av, int &ai, double &bv, int &bi) {
+    __shared__ double sav[NT / 64], sbv[NT / 64];
+    __shared__ int sai[NT / 64], sbi[NT / 64];
+    auto merge = [](double &v, int &i, double ov, int oi) {
+        if (ov > v || (ov == v && oi < i)) {
+            v = ov;
+            i = oi;
+        }
+    };
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double oa = __shfl_xor(av, o), ob = __shfl_xor(bv, o);
+        const int oai = __shfl_xor(ai, o), obi = __shfl_xor(bi, o);
+        merge(av, ai, oa, oai);
+        merge(bv, bi, ob, obi);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        sav[wave] = av;
+        sai[wave] = ai;
+        sbv[wave] = bv;
+        sbi[wave] = bi;
+    }
+    __syncthreads();
+    av = sav[0];
+    ai = sai[0];
+    bv = sbv[0];
+    bi = sbi[0];
+    for (int w = 1; w < NT / 64; w++) {
+        merge(av, ai, sav[w], sai[w]);
+        merge(bv, bi, sbv[w], sbi[w]);
+    }
+    __syncthreads();
+}
 
 // One workgroup of NT threads per template (NT = 1024 for small batches, whose
-// maps are large, 256 otherwise).
+// maps are large, 256 otherwise).  Three block reductions: the argmax of F
+// with the largest item reference Mref (log2 units), the lnBtSG sum, and the
+// two posterior argmaxes together.  Item k's partials are relative to
+// 2^(M_k) (M_k = -inf: no valid cell, partials 0), rescaled here by
+// exp((M_k - Mref) / LOG2E) <= 1, so no item reference needs maxF; the row
+// and column partials of a thread are read in one pass.
 template <bool BTSG, int NT>
 __global__ __launch_bounds__(NT) void finalize_kernel(
     const TmplDev *__restrict__ tds, const ItemPart *__restrict__ ipart,
@@ -1902,14 +1924,19 @@ __global__ __launch_bounds__(NT) void finalize_kernel(
     // together; the narrow one (many templates) is faster as the compiler
     // schedules it by itself
     constexpr bool FIN_UNROLL = NT >= 1024;
+    constexpr double INV_LOG2E = 1.0 / (double)LOG2E;
     const int t = blockIdx.x;
     const TmplDev td = tds[t];
     const int64_t nitems = (int64_t)td.n_rowblk * td.n_colblk;
     const ItemPart *ip = ipart + td.item_begin;
-    float bv = -INFINITY;
+    float bv = -INFINITY, Mref = -INFINITY;
     int bi = 0x7fffffff;
-    for (int64_t k = threadIdx.x; k < nitems; k += NT) argmax_merge(bv, bi, ip[k].maxF, ip[k].idx);
-    block_argmax<NT>(bv, bi);
+    for (int64_t k = threadIdx.x; k < nitems; k += NT) {
+        const ItemPart q = ip[k];
+        argmax_merge(bv, bi, q.maxF, q.idx);
+        Mref = fmaxf(Mref, q.M);
+    }
+    block_argmax_max<NT>(bv, bi, Mref);
     const float maxF = bv;
     const int64_t m_ML = bi / (int)td.N_tau;
     const int64_t n_ML = bi - m_ML * (int64_t)td.N_tau;
@@ -1917,67 +1944,67 @@ __global__ __launch_bounds__(NT) void finalize_kernel(
     double lnB = NAN, t0_MP = NAN, tau_MP = NAN;
     int64_t m_MP = -1, n_MP = -1;
     if (BTSG) {
-        // item k's partials are relative to exp(M_k / LOG2E), M_k in log2
-        // units and M_k / LOG2E <= maxF up to rounding
-        constexpr double INV_LOG2E = 1.0 / (double)LOG2E;
-        // M_k / LOG2E - maxF (FP64; small for the items that matter)
-        auto item_arg = [&](int64_t k) -> double {
-            return fma((double)ip[k].M, INV_LOG2E, -(double)maxF);
+        auto item_scale = [&](int64_t k) -> float {
+            const float Mk = ip[k].M;
+            return Mk == -INFINITY ? 0.0f : __expf((float)((double)(Mk - Mref) * INV_LOG2E));
         };
         double s = 0.0;
         for (int64_t k = threadIdx.x; k < nitems; k += NT) {
-            const double arg = item_arg(k);
-            if (k < FIN_SCALE_CAP) sh_scale[k] = __expf((float)arg);
-            s += (double)ip[k].S * exp(arg);
+            const float sc = item_scale(k);
+            if (k < FIN_SCALE_CAP) sh_scale[k] = sc;
+            s += (double)ip[k].S * (double)sc;
         }
         s = block_sum_d<NT>(s);  // contains a barrier: sh_scale is complete
-        lnB = log(70.0 / ((double)td.N_t0 * (double)td.N_tau)) + (double)maxF + log(s);
+        lnB = log(70.0 / ((double)td.N_t0 * (double)td.N_tau)) + (double)Mref * INV_LOG2E + log(s);
         auto scale = [&](int64_t k) -> float {
-            return k < FIN_SCALE_CAP ? sh_scale[k] : __expf((float)item_arg(k));
+            return k < FIN_SCALE_CAP ? sh_scale[k] : item_scale(k);
         };
-        double rv = -1.0;
-        int ri = 0x7fffffff;
-        for (int m = threadIdx.x; m < (int)td.N_t0; m += NT) {
-            const int rb = m / td.rb_rows;
-            const float *rp = rowpart + td.rowpart_off + m;  // [colblk][N_t0]
-            auto term = [&](uint32_t cb) {
-                return (double)(rp[(int64_t)cb * td.N_t0] * scale((int64_t)rb * td.n_colblk + cb));
-            };
-            double R = 0.0;
-            if constexpr (FIN_UNROLL) {
+        double rv = -1.0, cv = -1.0;
+        int ri = 0x7fffffff, ci = 0x7fffffff;
+        const int nmax = max((int)td.N_t0, (int)td.N_tau);
+        for (int i = threadIdx.x; i < nmax; i += NT) {
+            double R = -1.0, Cs = -1.0;
+            if (i < (int)td.N_t0) {  // row i: [colblk][N_t0]
+                const int rb = i / td.rb_rows;
+                const float *rp = rowpart + td.rowpart_off + i;
+                auto term = [&](uint32_t cb) {
+                    return (double)(rp[(int64_t)cb * td.N_t0] *
+                                    scale((int64_t)rb * td.n_colblk + cb));
+                };
+                R = 0.0;
+                if constexpr (FIN_UNROLL) {
 #pragma unroll 8
-                for (uint32_t cb = 0; cb < td.n_colblk; cb++) R += term(cb);
-            } else {
-                for (uint32_t cb = 0; cb < td.n_colblk; cb++) R += term(cb);
+                    for (uint32_t cb = 0; cb < td.n_colblk; cb++) R += term(cb);
+                } else {
+                    for (uint32_t cb = 0; cb < td.n_colblk; cb++) R += term(cb);
+                }
             }
-            if (R > rv || (R == rv && m < ri)) {
+            if (i < (int)td.N_tau) {  // column i: [rowblk][N_tau]
+                const int cb = i / TILE_COLS;
+                const float *cp = colpart + td.colpart_off + i;
+                auto term = [&](uint32_t rb) {
+                    return (double)(cp[(int64_t)rb * td.N_tau] *
+                                    scale((int64_t)rb * td.n_colblk + cb));
+                };
+                Cs = 0.0;
+                if constexpr (FIN_UNROLL) {
+#pragma unroll 8
+                    for (uint32_t rb = 0; rb < td.n_rowblk; rb++) Cs += term(rb);
+                } else {
+                    for (uint32_t rb = 0; rb < td.n_rowblk; rb++) Cs += term(rb);
+                }
+            }
+            if (R > rv) {  // i grows within a thread: the first maximum stays
                 rv = R;
-                ri = m;
+                ri = i;
             }
-        }
-        block_argmax_d<NT>(rv, ri);
-        m_MP = ri;
-        double cv = -1.0;
-        int ci = 0x7fffffff;
-        for (int n = threadIdx.x; n < (int)td.N_tau; n += NT) {
-            const int cb = n / TILE_COLS;
-            const float *cp = colpart + td.colpart_off + n;  // [rowblk][N_tau]
-            auto term = [&](uint32_t rb) {
-                return (double)(cp[(int64_t)rb * td.N_tau] * scale((int64_t)rb * td.n_colblk + cb));
-            };
-            double Cs = 0.0;
-            if constexpr (FIN_UNROLL) {
-#pragma unroll 8
-                for (uint32_t rb = 0; rb < td.n_rowblk; rb++) Cs += term(rb);
-            } else {
-                for (uint32_t rb = 0; rb < td.n_rowblk; rb++) Cs += term(rb);
-            }
-            if (Cs > cv || (Cs == cv && n < ci)) {
+            if (Cs > cv) {
                 cv = Cs;
-                ci = n;
+                ci = i;
             }
         }
-        block_argmax_d<NT>(cv, ci);
+        block_argmax2_d<NT>(rv, ri, cv, ci);
+        m_MP = ri;
         n_MP = ci;
         t0_MP = (double)td.o_t0 + ((double)m_MP + 0.5) * ((double)td.o_t0Band / (double)td.N_t0);
         tau_MP = (double)td.o_tau + ((double)n_MP + 0.5) * ((double)td.o_tauBand / (double)td.N_tau);
@@ -2613,9 +2640,12 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
     const bool strict = p.flags & TCW_FLAG_LAL_STRICT;
     auto *flags = (int *)p.d_flags.p;
     if (ev) TCW_HIP_TRY(hipEventRecord(ev[0], st));
-    hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(SR_WG), 0, st,
-                       (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
-                       cstride, (double *)p.d_prefix.p, flags);
+    // (stage_exp_kernel clears the single-atom flags of the exp templates, so
+    // a batch of exp templates only launches no stage_kernel)
+    if ((int64_t)p.exp_tmpls.size() < p.T)
+        hipLaunchKernelGGL(stage_kernel, dim3((unsigned)p.T), dim3(SR_WG), 0, st,
+                           (const TmplDev *)p.d_tmpl.p, (const SegDev *)p.d_segs.p, ts, cols,
+                           cstride, (double *)p.d_prefix.p, flags);
     if (!p.exp_tmpls.empty()) {
         const int64_t n_exp = (int64_t)p.exp_tmpls.size();
         const int64_t wide_blocks = (p.exp_max_N + 1023) / 1024;
@@ -2624,12 +2654,12 @@ int launch_all(tcw_plan &p, const uint32_t *ts, const float *cols, int64_t cstri
             hipLaunchKernelGGL(stage_exp_kernel<1>, dim3((unsigned)((p.exp_max_N + 255) / 256), gy),
                                dim3(256), 0, st, (const TmplDev *)p.d_tmpl.p,
                                (const int32_t *)p.d_exp_tmpls.p, n_exp, (const SegDev *)p.d_segs.p,
-                               ts, cols, cstride, (float *)p.d_merged.p);
+                               ts, cols, cstride, (float *)p.d_merged.p, flags);
         else if (n_exp > 0)
             hipLaunchKernelGGL(stage_exp_kernel<4>, dim3((unsigned)wide_blocks, gy), dim3(256), 0,
                                st, (const TmplDev *)p.d_tmpl.p, (const int32_t *)p.d_exp_tmpls.p,
                                n_exp, (const SegDev *)p.d_segs.p, ts, cols, cstride,
-                               (float *)p.d_merged.p);
+                               (float *)p.d_merged.p, flags);
     }
     // exp_init and exp_seg read the atom table and write different slots of the
     // E table: with enough blocks for both, exp_init runs on the side stream
