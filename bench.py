@@ -62,17 +62,18 @@ REFINE_METRIC = "refined candidate-steps/s on 1 GPU (2F + d2F/dtau per step)"
 # workloads; the commands and raw counters are in the named summaries.
 PROFILED_TRAFFIC = {
     "rect_batched": (1.98e9 + 26.18e9, "profiles/rect_fast_kernel.rect_batched.md"),
-    "exp_shared": (0.101e9 + 0.543e9, "profiles/exp_fast_kernel.exp_shared.md"),
+    "exp_shared": (0.156e9 + 0.540e9, "profiles/exp_fast_kernel.exp_shared.md"),
 }
 FP64_VALU_PEAK_TFLOPS = 78.6
 # What bounds the map kernel instead of HBM (rocprofv3 counters of the same
 # kernel, summarised in the profile files named here).
 LIMITER = {
-    "rect_batched": "FP64 VALU issue: ~46 VALU instructions per cell, SQ_ACTIVE_INST_VALU ~96-99% of "
-                    "SIMD cycles at the engine clock the card holds under this load (GRBM_GUI_ACTIVE: "
-                    "~1.8-1.9 GHz, not 2.4); profiles/rect_fast_kernel.rect_batched.md",
-    "exp_shared": "FP64 VALU issue (recurrence step + F per cell); "
-                  "profiles/exp_fast_kernel.exp_shared.md",
+    "rect_batched": "VALU issue: 47.25 VALU instructions per cell (22 FP64 add/mul/FMA), "
+                    "SQ_ACTIVE_INST_VALU ~99% of SIMD cycles at the ~1.9 GHz the card holds under "
+                    "this load; profiles/rect_fast_kernel.rect_batched.md",
+    "exp_shared": "latency at two waves per SIMD (one row chunk per column: 2048 waves): "
+                  "~56 VALU instructions per cell at ~59% VALU busy, FP64 dependency chains and "
+                  "LDS broadcasts of the entering atoms; profiles/exp_fast_kernel.exp_shared.md",
 }
 
 
