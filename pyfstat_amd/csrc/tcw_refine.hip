@@ -83,6 +83,7 @@ static_assert(sizeof(RItem) == 32, "RItem layout");
 
 struct Eval {
     double F, dFdt0, dFdtau;
+    double dFdlog;  // tau dF/dtau
     double g[NC];
 };
 
@@ -104,9 +105,14 @@ __device__ __forceinline__ double dpp_pair(double x) {
 }
 
 // One exchange step for all sums at once (the moves of one step read values
-// written a step earlier, so they need no wait states between them).
+// written a step earlier, so they need no wait states between them).  The
+// adds must not be contracted with a product that feeds them: a lane would
+// add its own unrounded product to its partner's rounded one, the two sides
+// of a pair would differ in the last bit, and the lanes of a group would walk
+// the ascent on different taus.
 template <int CTRL, int N>
 __device__ __forceinline__ void group_step(double (&s)[N]) {
+#pragma clang fp contract(off)
     double t[N];
 #pragma unroll
     for (int i = 0; i < N; i++) t[i] = dpp_pair<CTRL>(s[i]);
@@ -132,6 +138,7 @@ __device__ __forceinline__ void group_sum(double (&s)[N]) {
 struct LdsBins {
     const float4 *b;
     int base;
+    __device__ __forceinline__ int first() const { return base; }
     __device__ __forceinline__ void get(int k, float (&x)[NC]) const {
         const float4 p = b[2 * (k - base)], q = b[2 * (k - base) + 1];
         x[0] = p.x;
@@ -148,40 +155,71 @@ struct LdsBins {
 struct ColBins {
     const float *g;
     int64_t stride;
+    __device__ __forceinline__ int first() const { return 0; }
     __device__ __forceinline__ void get(int k, float (&x)[NC]) const {
 #pragma unroll
         for (int c = 0; c < NC; c++) x[c] = g[c * stride + k];
     }
 };
 
+// S_c and U_c from a lane's Horner sums: S_c = w0^{p_c} P_c, U_c = w0^{p_c}
+// ((u0 - G TA) P_c + G TA Z_c).
+__device__ __forceinline__ void horner_finish(const double (&P)[NC], const double (&Z)[NC],
+                                              double w0, double a, double b,
+                                              double (&s)[2 * NC]) {
+    const double w2 = w0 * w0;
+    const double bw1 = b * w0, bw2 = b * w2;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        s[c] = (c < 3 ? w2 : w0) * P[c];
+        s[NC + c] = fma(a, s[c], (c < 3 ? bw2 : bw1) * Z[c]);
+    }
+}
+
 // A lane's share of the window sums of its group's candidate: s[0..6] = S_c,
-// s[7..13] = U_c over the lane's bins; an empty support (kb > ke) gives zeros.
+// s[7..13] = U_c over the lane's bins k0 + G j (k0 = kb + gl, j = 0 .. n-1);
+// an empty share (k0 > ke) gives zeros.  With w_j = w0 r^j (r = exp(-G TA /
+// tau)) and rho_c = r^{p_c}, the lane's bins enter through two Horner sums
+// run from its last bin down to its first,
+//   P_c = sum_j rho_c^j x_{c,j},   Z_c = sum_j (j+1) rho_c^j x_{c,j},
+// (P <- rho P + x_j, Z <- rho Z + P): two FMAs a bin and column, no weight
+// recurrence.  Then S_c = w0^{p_c} P_c and, with u_j = u0 + G TA j,
+// U_c = w0^{p_c} (u0 P_c + G TA (Z_c - P_c)).
 template <int G, typename Bins>
 __device__ __forceinline__ void group_sums(const Bins &src, int kb, int ke, double t_data,
                                            double TA, double t0, double inv_tau, int gl,
                                            double (&s)[2 * NC]) {
-#pragma unroll
-    for (int i = 0; i < 2 * NC; i++) s[i] = 0.0;
     const int k0 = kb + gl;
-    double u = fma((double)k0, TA, t_data - t0);  // t_k - t0
-    double w = exp(-u * inv_tau);
-    const double rg = exp(-(double)G * TA * inv_tau);
-    const double du = (double)G * TA;
-    // two bins a trip: their LDS reads overlap
+    const double u0 = fma((double)k0, TA, t_data - t0);  // t_k0 - t0
+    const double GTA = (double)G * TA;
+    // an empty share keeps zeros even where its first weight would overflow
+    const bool any = k0 <= ke;
+    const double e0 = exp(-u0 * inv_tau);
+    const double w0 = any ? e0 : 0.0;
+    const double r1 = exp(-GTA * inv_tau);
+    const double r2 = r1 * r1;
+    // the lane's last bin opens the sums (P = Z = x); an empty share reads the
+    // source's first bin and keeps zeros
+    const int top = any ? k0 + G * ((ke - k0) / G) : src.first();
+    double P[NC], Z[NC];
+    {
+        float a[NC];
+        src.get(top, a);
+#pragma unroll
+        for (int c = 0; c < NC; c++) Z[c] = P[c] = (double)(any ? a[c] : 0.0f);
+    }
+    // then down to its first bin (lanes with fewer bins leave the loop early)
 #pragma unroll 2
-    for (int k = k0; k <= ke; k += G) {
+    for (int k = top - G; k >= k0; k -= G) {
         float a[NC];
         src.get(k, a);
-        const double w2 = w * w;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const double x = (c < 3 ? w2 : w) * (double)a[c];
-            s[c] += x;
-            s[NC + c] = fma(u, x, s[NC + c]);
+            P[c] = fma(c < 3 ? r2 : r1, P[c], (double)a[c]);
+            Z[c] = fma(c < 3 ? r2 : r1, Z[c], P[c]);
         }
-        w *= rg;
-        u += du;
     }
+    horner_finish(P, Z, w0, u0 - GTA, GTA, s);
 }
 
 // 1/x for a positive normal x (tau, and D where the cond test passes):
@@ -222,17 +260,24 @@ __device__ __forceinline__ Eval eval_from_sums(const double (&s)[2 * NC], double
     e.g[4] = 2.0 * (B * fai - C * fbi) * iD;
     e.g[5] = 2.0 * (A * fbr - C * far) * iD;
     e.g[6] = 2.0 * (A * fbi - C * fai) * iD;
-    double d0 = 0.0;
-    double d1[1] = {0.0};
+    // sum_c p_c X_c g_c as 2 (terms c < 3) + (terms c >= 3): two short chains
+    double d0a = S[0] * e.g[0], d0b = S[3] * e.g[3];
+    double d1a = s[NC] * e.g[0], d1b = s[NC + 3] * e.g[3];
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const double pc = c < 3 ? 2.0 : 1.0;
-        d0 = fma(pc * S[c], e.g[c], d0);
-        d1[0] = fma(pc * s[NC + c], e.g[c], d1[0]);
+    for (int c = 1; c < 3; c++) {
+        d0a = fma(S[c], e.g[c], d0a);
+        d1a = fma(s[NC + c], e.g[c], d1a);
     }
+#pragma unroll
+    for (int c = 4; c < NC; c++) {
+        d0b = fma(S[c], e.g[c], d0b);
+        d1b = fma(s[NC + c], e.g[c], d1b);
+    }
+    double d1[1] = {fma(2.0, d1a, d1b)};
     group_sum<G>(d1);
-    e.dFdt0 = d0 * inv_tau;
-    e.dFdtau = d1[0] * inv_tau * inv_tau;
+    e.dFdt0 = fma(2.0, d0a, d0b) * inv_tau;
+    e.dFdlog = d1[0] * inv_tau;
+    e.dFdtau = e.dFdlog * inv_tau;
     return e;
 }
 
@@ -255,17 +300,48 @@ struct Src {
     bool in_lds;
 };
 
-template <int G>
-__device__ __forceinline__ Eval eval_group(const Src &sr, const float *lds, int kb, int ke,
-                                           double t_data, double TA, double t0, double tau,
-                                           int gl) {
-    const double inv_tau = recip_pos(tau);
+template <int G, typename Bins>
+__device__ __forceinline__ Eval eval_bins(const Bins &src, int kb, int ke, double t_data,
+                                          double TA, double t0, double inv_tau, int gl) {
     double s[2 * NC];
-    if (sr.in_lds)  // block-uniform
-        group_sums<G>(LdsBins{(const float4 *)lds, sr.lbase}, kb, ke, t_data, TA, t0, inv_tau, gl, s);
-    else
-        group_sums<G>(ColBins{sr.g, sr.gstride}, kb, ke, t_data, TA, t0, inv_tau, gl, s);
+    group_sums<G>(src, kb, ke, t_data, TA, t0, inv_tau, gl, s);
     return eval_from_sums<G>(s, inv_tau);
+}
+
+struct Walk {
+    double tau, F;
+    int acc;
+};
+
+// The ascent of one candidate in s = log(tau): proposal s' = clamp(s + eta g,
+// lo, hi) with g = dF/ds, accepted when F rises (then eta *= 1.5, else eta *=
+// 0.5); a proposal that cannot move (zero gradient or clamped) is rejected like
+// a worse one.  ev(1/tau) evaluates the candidate; proposals are evaluated at
+// 1/tau = exp(-s') and the result's tau is exp(s) of the last accepted
+// proposal (tau0 itself when none was).  Every group evaluates every step
+// (no divergent skips).
+template <typename EvalFn>
+__device__ __forceinline__ Walk ascend(const EvalFn &ev, double tau0, double lo, double hi,
+                                       int n_steps, double step0) {
+    const Eval e0 = ev(recip_pos(tau0));
+    double s = log(tau0);
+    double F = e0.F, g = e0.dFdlog;
+    double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
+    int acc = 0;
+    for (int step = 0; step < n_steps; step++) {
+        const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
+        const bool moved = sn != s;
+        const Eval en = ev(exp(-sn));
+        const bool accept = moved && en.F > F;
+        if (accept) {
+            s = sn;
+            F = en.F;
+            g = en.dFdlog;
+            acc++;
+        }
+        eta *= accept ? 1.5 : 0.5;
+    }
+    return Walk{acc > 0 ? exp(s) : tau0, F, acc};
 }
 
 // Stage the item's support union into LDS when it fits (bin-major, 8 floats a
@@ -330,8 +406,11 @@ __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     const int gl = (int)threadIdx.x % G;
     const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
-    const double tau = sl.has ? tauv[sl.ci] : 1.0;
-    const Eval e = eval_group<G>(sr, lds, sl.kb, sl.ke, (double)tp.t_data, TA, t0, tau, gl);
+    const double itau = recip_pos(sl.has ? tauv[sl.ci] : 1.0);
+    const double t_data = (double)tp.t_data;
+    const Eval e = sr.in_lds  // block-uniform
+        ? eval_bins<G>(LdsBins{(const float4 *)lds, sr.lbase}, sl.kb, sl.ke, t_data, TA, t0, itau, gl)
+        : eval_bins<G>(ColBins{sr.g, sr.gstride}, sl.kb, sl.ke, t_data, TA, t0, itau, gl);
     if (!sl.has) return;
     if (gl == 0) {
         twoF[sl.ci] = 2.0 * e.F;
@@ -359,36 +438,25 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
     const double tau_in = sl.has ? tau_iv[sl.ci] : 1.0;
     const bool live = sl.has && tau_in > 0.0;
-    double tau = live ? tau_in : 1.0;
     const double lo = (live && tau_lo) ? log(tau_lo[sl.ci]) : -INFINITY;
     const double hi = (live && tau_hi) ? log(tau_hi[sl.ci]) : INFINITY;
     const int kb = live ? sl.kb : 1, ke = live ? sl.ke : 0;
-    Eval e = eval_group<G>(sr, lds, kb, ke, (double)tp.t_data, TA, t0, tau, gl);
-    double s = log(tau);
-    double g = tau * e.dFdtau;  // dF/dlog(tau)
-    double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
-    int acc = 0;
-    // every group evaluates every step (no divergent skips); a proposal that
-    // cannot move (zero gradient or clamped) is rejected like a worse one
-    for (int step = 0; step < n_steps; step++) {
-        const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
-        const bool moved = sn != s;
-        const double taun = moved ? exp(sn) : tau;
-        const Eval en = eval_group<G>(sr, lds, kb, ke, (double)tp.t_data, TA, t0, taun, gl);
-        const bool accept = moved && en.F > e.F;
-        if (accept) {
-            s = sn;
-            tau = taun;
-            e = en;
-            g = tau * e.dFdtau;
-            acc++;
-        }
-        eta *= accept ? 1.5 : 0.5;
+    const double t_data = (double)tp.t_data;
+    const double tau0 = live ? tau_in : 1.0;
+    Walk wk;
+    if (sr.in_lds) {  // block-uniform: one step loop per atom source
+        const LdsBins src{(const float4 *)lds, sr.lbase};
+        wk = ascend([&](double iv) { return eval_bins<G>(src, kb, ke, t_data, TA, t0, iv, gl); },
+                    tau0, lo, hi, n_steps, step0);
+    } else {
+        const ColBins src{sr.g, sr.gstride};
+        wk = ascend([&](double iv) { return eval_bins<G>(src, kb, ke, t_data, TA, t0, iv, gl); },
+                    tau0, lo, hi, n_steps, step0);
     }
     if (sl.has && gl == 0) {
-        tau_ov[sl.ci] = live ? tau : tau_in;
-        twoF[sl.ci] = live ? 2.0 * e.F : NAN;
-        if (n_accept) n_accept[sl.ci] = acc;
+        tau_ov[sl.ci] = live ? wk.tau : tau_in;
+        twoF[sl.ci] = live ? 2.0 * wk.F : NAN;
+        if (n_accept) n_accept[sl.ci] = wk.acc;
     }
 }
 
@@ -444,31 +512,33 @@ __device__ __forceinline__ void reg_slot(RegSlot<G> &r, int64_t blk, const RTmpl
     r.nw = wave_max_int(mine);
 }
 
+// The lane's sums in the Horner form of group_sums over its register bins
+// (positions past its share hold zeros, which add nothing at the top of a
+// Horner sum).
 template <int G>
 __device__ __forceinline__ Eval eval_regs(const RegSlot<G> &r, double t_data, double TA, double t0,
-                                          double tau) {
-    const double inv_tau = recip_pos(tau);
-    double s[2 * NC];
+                                          double inv_tau) {
+    const int k0 = r.kb + r.gl;
+    const double u0 = fma((double)k0, TA, t_data - t0);  // t_k0 - t0
+    const double GTA = (double)G * TA;
+    const double w0 = k0 <= r.ke ? exp(-u0 * inv_tau) : 0.0;
+    const double r1 = exp(-GTA * inv_tau);
+    const double r2 = r1 * r1;
+    double P[NC], Z[NC];
 #pragma unroll
-    for (int i = 0; i < 2 * NC; i++) s[i] = 0.0;
-    double u = fma((double)(r.kb + r.gl), TA, t_data - t0);  // t_k - t0
-    double w = exp(-u * inv_tau);
-    const double rg = exp(-(double)G * TA * inv_tau);
-    const double du = (double)G * TA;
+    for (int c = 0; c < NC; c++) P[c] = Z[c] = 0.0;
 #pragma unroll
-    for (int j = 0; j < RB; j++) {
+    for (int j = RB - 1; j >= 0; j--) {
         if (j < r.nw) {
-            const double w2 = w * w;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const double x = (c < 3 ? w2 : w) * (double)r.x[j][c];
-                s[c] += x;
-                s[NC + c] = fma(u, x, s[NC + c]);
+                P[c] = fma(c < 3 ? r2 : r1, P[c], (double)r.x[j][c]);
+                Z[c] = fma(c < 3 ? r2 : r1, Z[c], P[c]);
             }
-            w *= rg;
-            u += du;
         }
     }
+    double s[2 * NC];
+    horner_finish(P, Z, w0, u0 - GTA, GTA, s);
     return eval_from_sums<G>(s, inv_tau);
 }
 
@@ -485,7 +555,7 @@ __device__ __forceinline__ void forward_regs(
     reg_slot<G>(r, blk, tm, cand_t, list, n_list, k_lo, k_hi, cols, cstride, tp);
     const double t0 = r.has ? t0v[r.ci] : 0.0;
     const double tau = r.has ? tauv[r.ci] : 1.0;
-    const Eval e = eval_regs<G>(r, (double)tp.t_data, TA, t0, tau);
+    const Eval e = eval_regs<G>(r, (double)tp.t_data, TA, t0, recip_pos(tau));
     if (!r.has) return;
     if (r.gl == 0) {
         twoF[r.ci] = 2.0 * e.F;
@@ -510,7 +580,6 @@ __device__ __forceinline__ void ascent_regs(
     const double t0 = r.has ? t0v[r.ci] : 0.0;
     const double tau_in = r.has ? tau_iv[r.ci] : 1.0;
     const bool live = r.has && tau_in > 0.0;
-    double tau = live ? tau_in : 1.0;
     const double lo = (live && tau_lo) ? log(tau_lo[r.ci]) : -INFINITY;
     const double hi = (live && tau_hi) ? log(tau_hi[r.ci]) : INFINITY;
     if (!live) {  // an empty support: F = 2 and no move, as in the group kernel
@@ -521,30 +590,12 @@ __device__ __forceinline__ void ascent_regs(
 #pragma unroll
             for (int c = 0; c < NC; c++) r.x[j][c] = 0.0f;
     }
-    Eval e = eval_regs<G>(r, t_data, TA, t0, tau);
-    double s = log(tau);
-    double g = tau * e.dFdtau;  // dF/dlog(tau)
-    double eta = g != 0.0 ? step0 / fabs(g) : 0.0;
-    int acc = 0;
-    for (int step = 0; step < n_steps; step++) {
-        const double sn = fmin(fmax(fma(eta, g, s), lo), hi);
-        const bool moved = sn != s;
-        const double taun = moved ? exp(sn) : tau;
-        const Eval en = eval_regs<G>(r, t_data, TA, t0, taun);
-        const bool accept = moved && en.F > e.F;
-        if (accept) {
-            s = sn;
-            tau = taun;
-            e = en;
-            g = tau * e.dFdtau;
-            acc++;
-        }
-        eta *= accept ? 1.5 : 0.5;
-    }
+    const Walk wk = ascend([&](double it) { return eval_regs<G>(r, t_data, TA, t0, it); },
+                           live ? tau_in : 1.0, lo, hi, n_steps, step0);
     if (r.has && r.gl == 0) {
-        tau_ov[r.ci] = live ? tau : tau_in;
-        twoF[r.ci] = live ? 2.0 * e.F : NAN;
-        if (n_accept) n_accept[r.ci] = acc;
+        tau_ov[r.ci] = live ? wk.tau : tau_in;
+        twoF[r.ci] = live ? 2.0 * wk.F : NAN;
+        if (n_accept) n_accept[r.ci] = wk.acc;
     }
 }
 

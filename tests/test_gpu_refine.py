@@ -282,6 +282,35 @@ def test_ascent_monotone_bounded_and_converged(small):
     assert close >= 0.8 * prob.n
 
 
+def test_ascent_many_candidates_consistent(hip_ready):
+    """4096 grid maxima of every support class, 20 ascent steps: each
+    candidate's 2F is F at the tau it returns (the forward kernel there, and
+    the C oracle), i.e. the lanes of a multi-lane group walked one tau
+    together.  Groups whose lanes drift apart in the last bit of their shared
+    sums take different proposals, and after a few steps report an F that no
+    single tau gives."""
+    import torch
+
+    work = synth.workload_exp_refine(T=256, n_sft=1024, n_t0=128, n_tau=128, seed_index=43)
+    merged, _ = _merged(work)
+    prob, _ = _problem(work, merged, per_template=16)
+    L = prob.support_lengths
+    print(f"{prob.n} candidates, supports {L.min()}..{L.max()} bins", flush=True)
+    assert (L <= 8).sum() > 100 and ((L > 8) & (L <= 32)).sum() > 100 and (L > 32).sum() > 100
+    tau0 = prob.tau_init
+    lo = torch.clamp(tau0 - TA, min=float(TA))
+    hi = tau0 + TA
+    tau, twoF, nacc = prob.ascent(merged.cols, tau_lo=lo, tau_hi=hi, n_steps=20)
+    again = prob.forward_raw(merged.cols, prob.t0_init, tau, want_saved=False)[0]
+    twoF_h = twoF.cpu().numpy()
+    np.testing.assert_allclose(again.cpu().numpy(), twoF_h, rtol=1e-11)
+    Fo, _, _, _ = oracle.refine_eval_batch(merged.cols.cpu().numpy(), merged.m_off, merged.t_data,
+                                           TA, prob.cand_tmpl, prob.k_lo, prob.k_hi,
+                                           prob.t0_init.cpu().numpy(), tau.cpu().numpy())
+    np.testing.assert_allclose(twoF_h, 2 * Fo, rtol=1e-9)
+    assert nacc.cpu().numpy().max() >= 10  # long walks: the drift needs many steps
+
+
 def test_ascent_on_user_stream(small):
     """The ascent forks part of its work to the plan's own stream: on a torch
     stream of the caller's, it still starts after the work queued there (the
