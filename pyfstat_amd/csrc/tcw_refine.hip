@@ -38,8 +38,14 @@ namespace {
 
 constexpr int NC = TCW_NCOL;
 constexpr int RWG = 256;             // threads per block
-constexpr int LONG_G = 16;           // lanes per candidate in the LDS kernels
-constexpr int CAND_PER_ITEM = RWG / LONG_G;  // candidates per LDS item (one block)
+// Supports longer than SHORT4_MAX bins are walked by LONG_G lanes per
+// candidate, those longer than WIDE_MIN - 1 bins by WIDE_G lanes, so that no
+// lane walks much more than 25 bins per evaluation (the longest serial chains
+// of the launch).
+constexpr int LONG_G = 16;
+constexpr int WIDE_G = 32;
+constexpr int WIDE_MIN = 385;
+constexpr int CAND_PER_ITEM = RWG / LONG_G;  // most candidates of an LDS item (one block)
 constexpr int LDS_ATOMS_CAP = 1904;  // staged support bins: 32 B each, 60928 B
 constexpr int BWD_WG = 256;          // backward block: BWD_PER bins a thread,
 constexpr int BWD_PER = 4;           // BWD_WG apart (coalesced stores)
@@ -65,13 +71,23 @@ struct RTmpl {
     uint32_t t_data, N;
 };
 
+// A work item of the LDS kernels: up to CAND_PER_ITEM long-support
+// candidates of one or more templates, each template's support union (a
+// segment) staged in LDS, or a single candidate whose support does not fit,
+// read from global memory (lds_bins = 0).
 struct RItem {
-    int64_t t;      // template
-    int64_t cbeg;   // first slot in the plan's candidate order (long supports first)
-    int32_t ncand;  // candidates in the item
-    int32_t u_lo;   // first staged bin (relative to the template)
-    int32_t u_len;  // staged bins
-    int32_t in_lds; // 1: support union staged in LDS, 0: read from global memory
+    int64_t cbeg;      // first slot in the plan's candidate order
+    int32_t ncand;     // candidates in the item
+    int32_t seg_beg;   // first segment
+    int32_t nseg;      // segments
+    int32_t lds_bins;  // bins staged in LDS; 0: global item
+    int32_t g;         // lanes per candidate: LONG_G or WIDE_G
+    int32_t pad;
+};
+
+// A template's bins u_lo .. u_lo+u_len-1, staged from LDS bin lds_off on.
+struct RSeg {
+    int32_t t, u_lo, u_len, lds_off;
 };
 
 struct BItem {
@@ -120,12 +136,34 @@ __device__ __forceinline__ void group_step(double (&s)[N]) {
     for (int i = 0; i < N; i++) s[i] += t[i];
 }
 
-// Sum each value over the G lanes of an aligned group (G = 1, 4, 8, 16);
+// Lane l with lane l ^ 16 of its 32-lane half: ds_swizzle in bit-mask mode
+// (source lane = ((l & and) | or) ^ xor with and = 0x1F, or = 0, xor = 0x10).
+constexpr int SWZ_XOR16 = 0x1F | (0x10 << 10);
+
+__device__ __forceinline__ double swz_xor16(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)b, SWZ_XOR16);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), SWZ_XOR16);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int N>
+__device__ __forceinline__ void group_step_xor16(double (&s)[N]) {
+#pragma clang fp contract(off)
+    double t[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) t[i] = swz_xor16(s[i]);
+#pragma unroll
+    for (int i = 0; i < N; i++) s[i] += t[i];
+}
+
+// Sum each value over the G lanes of an aligned group (G = 1, 4, 8, 16, 32);
 // every lane gets the same totals.
 template <int G, int N>
 __device__ __forceinline__ void group_sum(double (&s)[N]) {
-    static_assert(G == 1 || G == 4 || G == 8 || G == 16, "group size");
-    if constexpr (G == 16) group_step<DPP_ROW_MIRROR>(s);
+    static_assert(G == 1 || G == 4 || G == 8 || G == 16 || G == 32, "group size");
+    if constexpr (G == 32) group_step_xor16(s);
+    if constexpr (G >= 16) group_step<DPP_ROW_MIRROR>(s);
     if constexpr (G >= 8) group_step<DPP_ROW_HALF_MIRROR>(s);
     if constexpr (G >= 4) {
         group_step<DPP_QUAD_XOR2>(s);
@@ -134,7 +172,7 @@ __device__ __forceinline__ void group_sum(double (&s)[N]) {
 }
 
 // Bin k's seven atoms from an item's LDS copy: bin-major, 8 floats a bin (two
-// 128-bit reads), bins from `base` on.
+// 128-bit reads), template-relative bin k at LDS bin k - base.
 struct LdsBins {
     const float4 *b;
     int base;
@@ -291,15 +329,6 @@ __device__ __forceinline__ void write_saved(double *__restrict__ row, const Eval
     row[2 + NC] = e.F;
 }
 
-// The atoms of an item: its LDS copy of the support union, or the template's
-// bins in global memory when the union does not fit.
-struct Src {
-    const float *g;  // global: template base
-    int64_t gstride;
-    int lbase;
-    bool in_lds;
-};
-
 template <int G, typename Bins>
 __device__ __forceinline__ Eval eval_bins(const Bins &src, int kb, int ke, double t_data,
                                           double TA, double t0, double inv_tau, int gl) {
@@ -344,73 +373,76 @@ __device__ __forceinline__ Walk ascend(const EvalFn &ev, double tau0, double lo,
     return Walk{acc > 0 ? exp(s) : tau0, F, acc};
 }
 
-// Stage the item's support union into LDS when it fits (bin-major, 8 floats a
-// bin; NT threads per block).
+// Stage an item's segments into LDS (bin-major, 8 floats a bin; NT threads
+// per block).  Global items stage nothing and need no barrier.
 template <int NT>
-__device__ __forceinline__ Src stage_support(const RItem &it, const RTmpl &tp,
-                                             const float *__restrict__ cols, int64_t cstride,
-                                             float *lds) {
-    Src sr;
-    sr.g = cols + tp.m_off;
-    sr.gstride = cstride;
-    sr.lbase = it.u_lo;
-    sr.in_lds = it.in_lds != 0;
-    if (sr.in_lds) {
-        const float *g = cols + tp.m_off + it.u_lo;
-        float4 *b = (float4 *)lds;
-        for (int k = threadIdx.x; k < it.u_len; k += NT) {
+__device__ __forceinline__ void stage_segments(const RItem &it, const RSeg *__restrict__ segs,
+                                               const RTmpl *__restrict__ tm,
+                                               const float *__restrict__ cols, int64_t cstride,
+                                               float *lds) {
+    if (it.lds_bins == 0) return;  // block-uniform
+    for (int q = 0; q < it.nseg; q++) {
+        const RSeg sg = segs[it.seg_beg + q];
+        const float *g = cols + tm[sg.t].m_off + sg.u_lo;
+        float4 *b = (float4 *)lds + 2 * sg.lds_off;
+        for (int k = threadIdx.x; k < sg.u_len; k += NT) {
             float x[NC];
 #pragma unroll
             for (int c = 0; c < NC; c++) x[c] = g[c * cstride + k];
             b[2 * k] = make_float4(x[0], x[1], x[2], x[3]);
             b[2 * k + 1] = make_float4(x[4], x[5], x[6], 0.0f);
         }
-        __syncthreads();
     }
-    return sr;
+    __syncthreads();
 }
 
-// Candidate slot of a lane: G-lane group j = thread / G handles item
-// candidate order[cbeg + j]; lanes without a candidate get an empty support.
+// Candidate slot of a lane: G-lane group j = thread / G handles the item's
+// candidate order[cbeg + j], whose template-relative bin k sits at LDS bin
+// k + shift[cbeg + j]; lanes without a candidate get an empty support.
 struct Slot {
     int64_t ci;
     bool has;
-    int kb, ke;
+    int kb, ke, shift;
+    RTmpl tp;
 };
 
 template <int G>
-__device__ __forceinline__ Slot lane_slot(const RItem &it, const int32_t *order,
-                                          const int32_t *k_lo, const int32_t *k_hi) {
+__device__ __forceinline__ Slot lane_slot(const RItem &it, const int32_t *__restrict__ order,
+                                          const int32_t *__restrict__ shift,
+                                          const int32_t *__restrict__ cand_t,
+                                          const RTmpl *__restrict__ tm,
+                                          const int32_t *__restrict__ k_lo,
+                                          const int32_t *__restrict__ k_hi) {
     const int j = (int)threadIdx.x / G;
     Slot sl;
     sl.has = j < it.ncand;
-    sl.ci = order[it.cbeg + (sl.has ? j : 0)];
+    const int64_t slot = it.cbeg + (sl.has ? j : 0);
+    sl.ci = order[slot];
+    sl.shift = shift[slot];
     sl.kb = sl.has ? k_lo[sl.ci] : 1;
     sl.ke = sl.has ? k_hi[sl.ci] : 0;
+    sl.tp = tm[cand_t[sl.ci]];
     return sl;
 }
 
 template <int G>
-__global__ __launch_bounds__(RWG) void refine_forward_kernel(
-    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ order,
-    const int32_t *__restrict__ k_lo,
+__device__ __forceinline__ void forward_block(
+    const RItem &it, const float *lds, const RTmpl *__restrict__ tm,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
+    const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
     const double *__restrict__ t0v, const double *__restrict__ tauv, double *__restrict__ twoF,
     double *__restrict__ lnB, double *__restrict__ saved) {
-    extern __shared__ float lds[];
-    const RItem it = items[blockIdx.x];
-    const RTmpl tp = tm[it.t];
-    const Src sr = stage_support<RWG>(it, tp, cols, cstride, lds);
-    // waves without a candidate leave after the staging barrier (no barrier follows)
+    // waves without a candidate leave (no barrier follows the staging one)
     if (((int)threadIdx.x & ~63) / G >= it.ncand) return;
     const int gl = (int)threadIdx.x % G;
-    const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
+    const Slot sl = lane_slot<G>(it, order, shift, cand_t, tm, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
     const double itau = recip_pos(sl.has ? tauv[sl.ci] : 1.0);
-    const double t_data = (double)tp.t_data;
-    const Eval e = sr.in_lds  // block-uniform
-        ? eval_bins<G>(LdsBins{(const float4 *)lds, sr.lbase}, sl.kb, sl.ke, t_data, TA, t0, itau, gl)
-        : eval_bins<G>(ColBins{sr.g, sr.gstride}, sl.kb, sl.ke, t_data, TA, t0, itau, gl);
+    const double t_data = (double)sl.tp.t_data;
+    const Eval e = it.lds_bins > 0  // block-uniform
+        ? eval_bins<G>(LdsBins{(const float4 *)lds, -sl.shift}, sl.kb, sl.ke, t_data, TA, t0, itau, gl)
+        : eval_bins<G>(ColBins{cols + sl.tp.m_off, cstride}, sl.kb, sl.ke, t_data, TA, t0, itau, gl);
     if (!sl.has) return;
     if (gl == 0) {
         twoF[sl.ci] = 2.0 * e.F;
@@ -419,37 +451,52 @@ __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     }
 }
 
+__global__ __launch_bounds__(RWG) void refine_forward_kernel(
+    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const RSeg *__restrict__ segs,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
+    const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, const double *__restrict__ tauv, double *__restrict__ twoF,
+    double *__restrict__ lnB, double *__restrict__ saved) {
+    extern __shared__ float lds[];
+    const RItem it = items[blockIdx.x];
+    stage_segments<RWG>(it, segs, tm, cols, cstride, lds);
+    if (it.g == WIDE_G)  // block-uniform
+        forward_block<WIDE_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+                              t0v, tauv, twoF, lnB, saved);
+    else
+        forward_block<LONG_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+                              t0v, tauv, twoF, lnB, saved);
+}
+
 template <int G>
-__global__ __launch_bounds__(RWG) void refine_ascent_kernel(
-    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const int32_t *__restrict__ order,
-    const int32_t *__restrict__ k_lo,
+__device__ __forceinline__ void ascent_block(
+    const RItem &it, const float *lds, const RTmpl *__restrict__ tm,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
+    const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
     const double *__restrict__ t0v, const double *tau_iv, double *tau_ov,
     const double *__restrict__ tau_lo, const double *__restrict__ tau_hi, int n_steps,
     double step0, double *__restrict__ twoF, int32_t *__restrict__ n_accept) {
-    extern __shared__ float lds[];
-    const RItem it = items[blockIdx.x];
-    const RTmpl tp = tm[it.t];
-    const Src sr = stage_support<RWG>(it, tp, cols, cstride, lds);
-    // waves without a candidate leave after the staging barrier (no barrier follows)
+    // waves without a candidate leave (no barrier follows the staging one)
     if (((int)threadIdx.x & ~63) / G >= it.ncand) return;
     const int gl = (int)threadIdx.x % G;
-    const Slot sl = lane_slot<G>(it, order, k_lo, k_hi);
+    const Slot sl = lane_slot<G>(it, order, shift, cand_t, tm, k_lo, k_hi);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
     const double tau_in = sl.has ? tau_iv[sl.ci] : 1.0;
     const bool live = sl.has && tau_in > 0.0;
     const double lo = (live && tau_lo) ? log(tau_lo[sl.ci]) : -INFINITY;
     const double hi = (live && tau_hi) ? log(tau_hi[sl.ci]) : INFINITY;
     const int kb = live ? sl.kb : 1, ke = live ? sl.ke : 0;
-    const double t_data = (double)tp.t_data;
+    const double t_data = (double)sl.tp.t_data;
     const double tau0 = live ? tau_in : 1.0;
     Walk wk;
-    if (sr.in_lds) {  // block-uniform: one step loop per atom source
-        const LdsBins src{(const float4 *)lds, sr.lbase};
+    if (it.lds_bins > 0) {  // block-uniform: one step loop per atom source
+        const LdsBins src{(const float4 *)lds, -sl.shift};
         wk = ascend([&](double iv) { return eval_bins<G>(src, kb, ke, t_data, TA, t0, iv, gl); },
                     tau0, lo, hi, n_steps, step0);
     } else {
-        const ColBins src{sr.g, sr.gstride};
+        const ColBins src{cols + sl.tp.m_off, cstride};
         wk = ascend([&](double iv) { return eval_bins<G>(src, kb, ke, t_data, TA, t0, iv, gl); },
                     tau0, lo, hi, n_steps, step0);
     }
@@ -458,6 +505,25 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
         twoF[sl.ci] = live ? 2.0 * wk.F : NAN;
         if (n_accept) n_accept[sl.ci] = wk.acc;
     }
+}
+
+__global__ __launch_bounds__(RWG) void refine_ascent_kernel(
+    const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const RSeg *__restrict__ segs,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
+    const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
+    const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
+    const double *__restrict__ t0v, const double *tau_iv, double *tau_ov,
+    const double *__restrict__ tau_lo, const double *__restrict__ tau_hi, int n_steps,
+    double step0, double *__restrict__ twoF, int32_t *__restrict__ n_accept) {
+    extern __shared__ float lds[];
+    const RItem it = items[blockIdx.x];
+    stage_segments<RWG>(it, segs, tm, cols, cstride, lds);
+    if (it.g == WIDE_G)  // block-uniform
+        ascent_block<WIDE_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+                             t0v, tau_iv, tau_ov, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
+    else
+        ascent_block<LONG_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+                             t0v, tau_iv, tau_ov, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
 }
 
 // ---- short supports in registers: G lanes per candidate (G = 1 or 4) ----
@@ -738,12 +804,12 @@ struct tcw_refine_plan {
     uint32_t TAtom = 0;
     int lds_bytes = 0;
     int64_t n_items = 0, n_bitems = 0, n_long = 0, n_mid = 0;
-    // d_order: candidate indices; long supports (grouped by template, in
-    // caller order, indexed by the items), then supports of
-    // SHORT1_MAX+1 .. SHORT4_MAX bins (four lanes each), then those of up to
-    // SHORT1_MAX bins (one lane each), the last two classes by decreasing
-    // length
-    DevBuf d_tmpl, d_items, d_bitems, d_klo, d_khi, d_ctmpl, d_coef, d_order;
+    // d_order: candidate indices; long supports item by item (within an item
+    // by decreasing length), then supports of SHORT1_MAX+1 .. SHORT4_MAX
+    // bins (four lanes each), then those of up to SHORT1_MAX bins (one lane
+    // each), the last two classes by decreasing length.  d_shift: for each
+    // long slot, LDS bin minus template bin of its item's copy.
+    DevBuf d_tmpl, d_items, d_segs, d_shift, d_bitems, d_klo, d_khi, d_ctmpl, d_coef, d_order;
     // the register kernels run on `side`, beside the LDS kernel on the
     // caller's stream, so their blocks fill the LDS kernel's tail
     hipStream_t side = nullptr;
@@ -757,6 +823,8 @@ struct tcw_refine_plan {
         d_coef.release();
         d_tmpl.release();
         d_items.release();
+        d_segs.release();
+        d_shift.release();
         d_bitems.release();
         d_klo.release();
         d_khi.release();
@@ -764,6 +832,103 @@ struct tcw_refine_plan {
 };
 
 namespace {
+
+// Consecutive long-support candidates of one template, longs[first ..
+// first+n), and the union of their supports, template bins lo .. lo+len-1.
+struct Chunk {
+    int32_t t, first, n, lo, len;
+    int32_t g;  // lanes per candidate
+};
+
+// Pack chunks into items of one lane count each.  A chunk whose union fits in
+// LDS goes (chunks with more candidates first) into the open item of its lane
+// count with the fewest free candidate slots that still takes its candidates
+// (RWG / g a block) and bins (LDS_ATOMS_CAP), looking at the latest item of
+// each free-slot count only (linear time); a chunk that does not fit in LDS
+// is an item of its own, read from global memory.  Items run by decreasing total
+// support bins (the heaviest blocks start first); within an item the
+// candidates run by decreasing length, so the groups of a wave walk similar
+// bin counts.  Appends the items, their segments, and the long candidates'
+// order and LDS shifts; returns the most bins an item stages.
+int pack_items(const std::vector<Chunk> &chunks, const std::vector<int32_t> &longs,
+               const int32_t *k_lo, const int32_t *k_hi, std::vector<RItem> &items,
+               std::vector<RSeg> &segs, std::vector<int32_t> &order,
+               std::vector<int32_t> &shifts) {
+    struct Pack {
+        std::vector<int32_t> chunk;
+        int ncand = 0, bins = 0;  // bins = 0: a global item
+        int g = LONG_G;
+        int64_t work = 0;
+    };
+    const auto len = [&](int32_t c) { return k_hi[c] - k_lo[c] + 1; };
+    std::vector<int32_t> ids(chunks.size());
+    for (size_t i = 0; i < ids.size(); i++) ids[i] = (int32_t)i;
+    std::stable_sort(ids.begin(), ids.end(),
+                     [&](int32_t a, int32_t b) { return chunks[a].n > chunks[b].n; });
+    std::vector<Pack> packs;
+    // open items by lane count (0: LONG_G, 1: WIDE_G) and free slots
+    std::vector<std::vector<int32_t>> open[2];
+    open[0].resize((size_t)CAND_PER_ITEM + 1);
+    open[1].resize((size_t)CAND_PER_ITEM + 1);
+    for (const int32_t id : ids) {
+        const Chunk &c = chunks[(size_t)id];
+        const int cap = RWG / c.g;
+        std::vector<std::vector<int32_t>> &op = open[c.g == WIDE_G ? 1 : 0];
+        int64_t w = 0;
+        for (int k = 0; k < c.n; k++) w += len(longs[(size_t)(c.first + k)]);
+        const bool in_lds = c.len <= LDS_ATOMS_CAP;
+        int32_t di = -1;
+        for (int f = c.n; in_lds && di < 0 && f <= cap; f++) {
+            std::vector<int32_t> &st = op[(size_t)f];
+            if (!st.empty() && packs[(size_t)st.back()].bins + c.len <= LDS_ATOMS_CAP) {
+                di = st.back();
+                st.pop_back();
+            }
+        }
+        if (di < 0) {
+            di = (int32_t)packs.size();
+            packs.emplace_back();
+            packs.back().g = c.g;
+        }
+        Pack &q = packs[(size_t)di];
+        q.chunk.push_back(id);
+        q.ncand += c.n;
+        q.bins += in_lds ? c.len : 0;
+        q.work += w;
+        if (in_lds && q.ncand < cap) op[(size_t)(cap - q.ncand)].push_back(di);
+    }
+    std::stable_sort(packs.begin(), packs.end(),
+                     [](const Pack &a, const Pack &b) { return a.work > b.work; });
+    int max_bins = 0;
+    std::vector<std::pair<int32_t, int32_t>> cs;  // (candidate, shift)
+    for (const Pack &q : packs) {
+        RItem it{};
+        it.cbeg = (int64_t)order.size();
+        it.ncand = q.ncand;
+        it.seg_beg = (int32_t)segs.size();
+        it.nseg = (int32_t)q.chunk.size();
+        it.lds_bins = q.bins;
+        it.g = q.g;
+        cs.clear();
+        int off = 0;
+        for (const int32_t id : q.chunk) {
+            const Chunk &c = chunks[(size_t)id];
+            segs.push_back(RSeg{c.t, c.lo, c.len, off});
+            for (int k = 0; k < c.n; k++) cs.emplace_back(longs[(size_t)(c.first + k)], off - c.lo);
+            off += c.len;
+        }
+        std::stable_sort(cs.begin(), cs.end(),
+                         [&](const std::pair<int32_t, int32_t> &a,
+                             const std::pair<int32_t, int32_t> &b) { return len(a.first) > len(b.first); });
+        for (const auto &x : cs) {
+            order.push_back(x.first);
+            shifts.push_back(q.bins > 0 ? x.second : 0);
+        }
+        max_bins = std::max(max_bins, q.bins);
+        items.push_back(it);
+    }
+    return max_bins;
+}
 
 // blocks for n candidates at G lanes each
 unsigned reg_blocks(int64_t n, int G) { return (unsigned)((n * G + RWG - 1) / RWG); }
@@ -807,10 +972,9 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     if (d->m_off[0] < 0 || d->cand_off[0] != 0 || d->cand_off[d->T] != d->n_cand)
         return fail(TCW_ERR_INVALID, "cand_off must run from 0 to n_cand");
     std::vector<RTmpl> tm((size_t)d->T);
-    std::vector<RItem> items;
-    std::vector<int32_t> order, mids, shorts;
+    std::vector<int32_t> longs, mids, shorts;
+    std::vector<Chunk> chunks;
     std::vector<BItem> bitems;
-    int max_len = 0;
     for (int64_t t = 0; t < d->T; t++) {
         const int64_t N = d->m_off[t + 1] - d->m_off[t];
         if (N <= 0 || N > INT32_MAX / 2)
@@ -825,41 +989,45 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
             if (d->k_lo[i] < 0 || d->k_hi[i] < d->k_lo[i] || d->k_hi[i] >= N)
                 return fail(TCW_ERR_INVALID, "candidate " + std::to_string(i) +
                                                  ": support outside its template's bins");
-        // short supports go to the register lists; items take consecutive
-        // long ones while their union fits in LDS
-        const int64_t l0 = (int64_t)order.size();
-        for (int64_t i = c0; i < c1; i++) {
-            const int len = d->k_hi[i] - d->k_lo[i] + 1;
-            if (len <= SHORT1_MAX)
-                shorts.push_back((int32_t)i);
-            else if (len <= SHORT4_MAX)
-                mids.push_back((int32_t)i);
-            else
-                order.push_back((int32_t)i);
-        }
-        const int64_t l1 = (int64_t)order.size();
-        int64_t i = l0;
-        while (i < l1) {
-            int lo = d->k_lo[order[(size_t)i]], hi = d->k_hi[order[(size_t)i]];
-            int64_t j = i + 1;
-            while (j < l1 && j - i < CAND_PER_ITEM) {
-                const int32_t cj = order[(size_t)j];
-                const int nlo = std::min(lo, d->k_lo[cj]), nhi = std::max(hi, d->k_hi[cj]);
-                if (nhi - nlo + 1 > LDS_ATOMS_CAP && hi - lo + 1 <= LDS_ATOMS_CAP) break;
-                lo = nlo;
-                hi = nhi;
-                j++;
+        // short supports go to the register lists; the long ones of a
+        // template, by lane count, form chunks of consecutive candidates
+        // whose union fits in LDS (a support too long for LDS starts a chunk
+        // read from global memory)
+        for (const int g : {LONG_G, WIDE_G}) {
+            const size_t l0 = longs.size();
+            for (int64_t i = c0; i < c1; i++) {
+                const int len = d->k_hi[i] - d->k_lo[i] + 1;
+                if (g == LONG_G && len <= SHORT1_MAX)
+                    shorts.push_back((int32_t)i);
+                else if (g == LONG_G && len <= SHORT4_MAX)
+                    mids.push_back((int32_t)i);
+                else if (len > SHORT4_MAX && (len >= WIDE_MIN) == (g == WIDE_G))
+                    longs.push_back((int32_t)i);
             }
-            const int len = hi - lo + 1;
-            const int in_lds = len <= LDS_ATOMS_CAP ? 1 : 0;
-            items.push_back(RItem{t, i, (int32_t)(j - i), lo, len, in_lds});
-            if (in_lds) max_len = std::max(max_len, len);
-            i = j;
+            size_t i = l0;
+            while (i < longs.size()) {
+                int lo = d->k_lo[longs[i]], hi = d->k_hi[longs[i]];
+                size_t j = i + 1;
+                while (j < longs.size() && (int)(j - i) < RWG / g) {
+                    const int32_t cj = longs[j];
+                    const int nlo = std::min(lo, d->k_lo[cj]), nhi = std::max(hi, d->k_hi[cj]);
+                    if (nhi - nlo + 1 > LDS_ATOMS_CAP && hi - lo + 1 <= LDS_ATOMS_CAP) break;
+                    lo = nlo;
+                    hi = nhi;
+                    j++;
+                }
+                chunks.push_back(Chunk{(int32_t)t, (int32_t)i, (int32_t)(j - i), lo, hi - lo + 1, g});
+                i = j;
+            }
         }
         // every bin gets a backward slot (templates without candidates get zeros)
         for (int64_t b = 0; b < N; b += BWD_BINS)
             bitems.push_back(BItem{t, (int32_t)b, (int32_t)std::min<int64_t>(BWD_BINS, N - b)});
     }
+    std::vector<RItem> items;
+    std::vector<RSeg> segs;
+    std::vector<int32_t> order, shifts;
+    const int max_bins = pack_items(chunks, longs, d->k_lo, d->k_hi, items, segs, order, shifts);
     const int device = tcwc::handle_device(h);
     int rc = set_device(device);
     if (rc) return rc;
@@ -869,11 +1037,7 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     p->n_cand = d->n_cand;
     p->TAtom = d->TAtom;
     p->total_bins = d->m_off[d->T];
-    p->lds_bytes = (int)(max_len * 2 * sizeof(float4));
-    // widest unions first: their waves walk the longest serial chains, and
-    // blocks start roughly in index order
-    std::stable_sort(items.begin(), items.end(),
-                     [](const RItem &a, const RItem &b) { return a.u_len > b.u_len; });
+    p->lds_bytes = (int)(max_bins * 2 * sizeof(float4));
     p->n_items = (int64_t)items.size();
     p->n_bitems = (int64_t)bitems.size();
     p->n_long = (int64_t)order.size();
@@ -889,6 +1053,8 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     const size_t nc = (size_t)std::max<int64_t>(d->n_cand, 1);
     if ((rc = p->d_tmpl.ensure(tm.size() * sizeof(RTmpl))) ||
         (rc = p->d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(RItem))) ||
+        (rc = p->d_segs.ensure(std::max<size_t>(segs.size(), 1) * sizeof(RSeg))) ||
+        (rc = p->d_shift.ensure(std::max<size_t>(shifts.size(), 1) * sizeof(int32_t))) ||
         (rc = p->d_bitems.ensure(bitems.size() * sizeof(BItem))) ||
         (rc = p->d_klo.ensure(nc * sizeof(int32_t))) || (rc = p->d_khi.ensure(nc * sizeof(int32_t))) ||
         (rc = p->d_ctmpl.ensure(nc * sizeof(int32_t))) || (rc = p->d_coef.ensure(nc * sizeof(CandCoef))) ||
@@ -903,6 +1069,11 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
         e = hipMemcpy(p->d_tmpl.p, tm.data(), tm.size() * sizeof(RTmpl), hipMemcpyHostToDevice);
     if (e == hipSuccess && !items.empty())
         e = hipMemcpy(p->d_items.p, items.data(), items.size() * sizeof(RItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !segs.empty())
+        e = hipMemcpy(p->d_segs.p, segs.data(), segs.size() * sizeof(RSeg), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !shifts.empty())
+        e = hipMemcpy(p->d_shift.p, shifts.data(), shifts.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_bitems.p, bitems.data(), bitems.size() * sizeof(BItem),
                       hipMemcpyHostToDevice);
@@ -945,9 +1116,11 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
     // one evaluation: both launches on st (a fork and join to the side stream
     // costs more than their overlap saves)
     if (p->n_items > 0)
-        hipLaunchKernelGGL(refine_forward_kernel<LONG_G>, dim3((unsigned)p->n_items),
+        hipLaunchKernelGGL(refine_forward_kernel, dim3((unsigned)p->n_items),
                            dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
-                           (const RItem *)p->d_items.p, order, (const int32_t *)p->d_klo.p,
+                           (const RItem *)p->d_items.p, (const RSeg *)p->d_segs.p, order,
+                           (const int32_t *)p->d_shift.p, (const int32_t *)p->d_ctmpl.p,
+                           (const int32_t *)p->d_klo.p,
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
                            twoF, lnBtSG, saved);
     const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
@@ -1003,9 +1176,11 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
     hipStream_t side;
     if ((rc = fork_side(p, st, &side))) return rc;
     if (p->n_items > 0)
-        hipLaunchKernelGGL(refine_ascent_kernel<LONG_G>, dim3((unsigned)p->n_items),
+        hipLaunchKernelGGL(refine_ascent_kernel, dim3((unsigned)p->n_items),
                            dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
-                           (const RItem *)p->d_items.p, order, (const int32_t *)p->d_klo.p,
+                           (const RItem *)p->d_items.p, (const RSeg *)p->d_segs.p, order,
+                           (const int32_t *)p->d_shift.p, (const int32_t *)p->d_ctmpl.p,
+                           (const int32_t *)p->d_klo.p,
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0,
                            tau_in, tau_out, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
     const int64_t n_short = p->n_cand - p->n_long - p->n_mid;
