@@ -46,7 +46,7 @@ constexpr int LONG_G = 16;
 constexpr int WIDE_G = 32;
 constexpr int WIDE_MIN = 385;
 constexpr int CAND_PER_ITEM = RWG / LONG_G;  // most candidates of an LDS item (one block)
-constexpr int LDS_ATOMS_CAP = 1904;  // staged support bins: 32 B each, 60928 B
+constexpr int LDS_ATOMS_CAP = 1696;  // staged bins: 32 B each, 54272 B (three blocks a CU)
 constexpr int BWD_WG = 256;          // backward block: BWD_PER bins a thread,
 constexpr int BWD_PER = 4;           // BWD_WG apart (coalesced stores)
 constexpr int BWD_BINS = BWD_WG * BWD_PER;  // bins per backward block
@@ -171,14 +171,16 @@ __device__ __forceinline__ void group_sum(double (&s)[N]) {
     }
 }
 
-// Bin k's seven atoms from an item's LDS copy: bin-major, 8 floats a bin (two
-// 128-bit reads), template-relative bin k at LDS bin k - base.
+// Bin k's seven atoms from an item's LDS copy: two planes of float4 (columns
+// 0-3, then columns 4-6 and a pad), `plane` float4s apart, so the lanes of a
+// group read consecutive 16-byte words; template-relative bin k at LDS bin
+// k - base.
 struct LdsBins {
     const float4 *b;
-    int base;
+    int base, plane;
     __device__ __forceinline__ int first() const { return base; }
     __device__ __forceinline__ void get(int k, float (&x)[NC]) const {
-        const float4 p = b[2 * (k - base)], q = b[2 * (k - base) + 1];
+        const float4 p = b[k - base], q = b[plane + k - base];
         x[0] = p.x;
         x[1] = p.y;
         x[2] = p.z;
@@ -373,24 +375,25 @@ __device__ __forceinline__ Walk ascend(const EvalFn &ev, double tau0, double lo,
     return Walk{acc > 0 ? exp(s) : tau0, F, acc};
 }
 
-// Stage an item's segments into LDS (bin-major, 8 floats a bin; NT threads
-// per block).  Global items stage nothing and need no barrier.
+// Stage an item's segments into LDS (the two planes of LdsBins, `plane`
+// float4s apart; NT threads per block).  Global items stage nothing and need
+// no barrier.
 template <int NT>
 __device__ __forceinline__ void stage_segments(const RItem &it, const RSeg *__restrict__ segs,
                                                const RTmpl *__restrict__ tm,
                                                const float *__restrict__ cols, int64_t cstride,
-                                               float *lds) {
+                                               float *lds, int plane) {
     if (it.lds_bins == 0) return;  // block-uniform
     for (int q = 0; q < it.nseg; q++) {
         const RSeg sg = segs[it.seg_beg + q];
         const float *g = cols + tm[sg.t].m_off + sg.u_lo;
-        float4 *b = (float4 *)lds + 2 * sg.lds_off;
+        float4 *b = (float4 *)lds + sg.lds_off;
         for (int k = threadIdx.x; k < sg.u_len; k += NT) {
             float x[NC];
 #pragma unroll
             for (int c = 0; c < NC; c++) x[c] = g[c * cstride + k];
-            b[2 * k] = make_float4(x[0], x[1], x[2], x[3]);
-            b[2 * k + 1] = make_float4(x[4], x[5], x[6], 0.0f);
+            b[k] = make_float4(x[0], x[1], x[2], x[3]);
+            b[plane + k] = make_float4(x[4], x[5], x[6], 0.0f);
         }
     }
     __syncthreads();
@@ -427,7 +430,7 @@ __device__ __forceinline__ Slot lane_slot(const RItem &it, const int32_t *__rest
 
 template <int G>
 __device__ __forceinline__ void forward_block(
-    const RItem &it, const float *lds, const RTmpl *__restrict__ tm,
+    const RItem &it, const float *lds, int plane, const RTmpl *__restrict__ tm,
     const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
     const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
@@ -441,7 +444,7 @@ __device__ __forceinline__ void forward_block(
     const double itau = recip_pos(sl.has ? tauv[sl.ci] : 1.0);
     const double t_data = (double)sl.tp.t_data;
     const Eval e = it.lds_bins > 0  // block-uniform
-        ? eval_bins<G>(LdsBins{(const float4 *)lds, -sl.shift}, sl.kb, sl.ke, t_data, TA, t0, itau, gl)
+        ? eval_bins<G>(LdsBins{(const float4 *)lds, -sl.shift, plane}, sl.kb, sl.ke, t_data, TA, t0, itau, gl)
         : eval_bins<G>(ColBins{cols + sl.tp.m_off, cstride}, sl.kb, sl.ke, t_data, TA, t0, itau, gl);
     if (!sl.has) return;
     if (gl == 0) {
@@ -453,6 +456,7 @@ __device__ __forceinline__ void forward_block(
 
 __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const RSeg *__restrict__ segs,
+    int plane,
     const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
     const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
@@ -460,18 +464,18 @@ __global__ __launch_bounds__(RWG) void refine_forward_kernel(
     double *__restrict__ lnB, double *__restrict__ saved) {
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
-    stage_segments<RWG>(it, segs, tm, cols, cstride, lds);
+    stage_segments<RWG>(it, segs, tm, cols, cstride, lds, plane);
     if (it.g == WIDE_G)  // block-uniform
-        forward_block<WIDE_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+        forward_block<WIDE_G>(it, lds, plane, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
                               t0v, tauv, twoF, lnB, saved);
     else
-        forward_block<LONG_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+        forward_block<LONG_G>(it, lds, plane, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
                               t0v, tauv, twoF, lnB, saved);
 }
 
 template <int G>
 __device__ __forceinline__ void ascent_block(
-    const RItem &it, const float *lds, const RTmpl *__restrict__ tm,
+    const RItem &it, const float *lds, int plane, const RTmpl *__restrict__ tm,
     const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
     const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
@@ -492,7 +496,7 @@ __device__ __forceinline__ void ascent_block(
     const double tau0 = live ? tau_in : 1.0;
     Walk wk;
     if (it.lds_bins > 0) {  // block-uniform: one step loop per atom source
-        const LdsBins src{(const float4 *)lds, -sl.shift};
+        const LdsBins src{(const float4 *)lds, -sl.shift, plane};
         wk = ascend([&](double iv) { return eval_bins<G>(src, kb, ke, t_data, TA, t0, iv, gl); },
                     tau0, lo, hi, n_steps, step0);
     } else {
@@ -509,6 +513,7 @@ __device__ __forceinline__ void ascent_block(
 
 __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     const RTmpl *__restrict__ tm, const RItem *__restrict__ items, const RSeg *__restrict__ segs,
+    int plane,
     const int32_t *__restrict__ order, const int32_t *__restrict__ shift,
     const int32_t *__restrict__ cand_t, const int32_t *__restrict__ k_lo,
     const int32_t *__restrict__ k_hi, const float *__restrict__ cols, int64_t cstride, double TA,
@@ -517,12 +522,12 @@ __global__ __launch_bounds__(RWG) void refine_ascent_kernel(
     double step0, double *__restrict__ twoF, int32_t *__restrict__ n_accept) {
     extern __shared__ float lds[];
     const RItem it = items[blockIdx.x];
-    stage_segments<RWG>(it, segs, tm, cols, cstride, lds);
+    stage_segments<RWG>(it, segs, tm, cols, cstride, lds, plane);
     if (it.g == WIDE_G)  // block-uniform
-        ascent_block<WIDE_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+        ascent_block<WIDE_G>(it, lds, plane, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
                              t0v, tau_iv, tau_ov, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
     else
-        ascent_block<LONG_G>(it, lds, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
+        ascent_block<LONG_G>(it, lds, plane, tm, order, shift, cand_t, k_lo, k_hi, cols, cstride, TA,
                              t0v, tau_iv, tau_ov, tau_lo, tau_hi, n_steps, step0, twoF, n_accept);
 }
 
@@ -802,7 +807,7 @@ struct tcw_refine_plan {
     int device = 0;
     int64_t T = 0, n_cand = 0, total_bins = 0;
     uint32_t TAtom = 0;
-    int lds_bytes = 0;
+    int lds_bytes = 0, lds_plane = 0;  // LDS per block; float4s between the planes
     int64_t n_items = 0, n_bitems = 0, n_long = 0, n_mid = 0;
     // d_order: candidate indices; long supports item by item (within an item
     // by decreasing length), then supports of SHORT1_MAX+1 .. SHORT4_MAX
@@ -1037,6 +1042,7 @@ int tcw_refine_plan_create(tcw_handle *h, const tcw_refine_desc *d, tcw_refine_p
     p->n_cand = d->n_cand;
     p->TAtom = d->TAtom;
     p->total_bins = d->m_off[d->T];
+    p->lds_plane = max_bins;
     p->lds_bytes = (int)(max_bins * 2 * sizeof(float4));
     p->n_items = (int64_t)items.size();
     p->n_bitems = (int64_t)bitems.size();
@@ -1118,7 +1124,7 @@ int tcw_refine_forward(tcw_refine_plan *p, const float *cols, int64_t col_stride
     if (p->n_items > 0)
         hipLaunchKernelGGL(refine_forward_kernel, dim3((unsigned)p->n_items),
                            dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
-                           (const RItem *)p->d_items.p, (const RSeg *)p->d_segs.p, order,
+                           (const RItem *)p->d_items.p, (const RSeg *)p->d_segs.p, p->lds_plane, order,
                            (const int32_t *)p->d_shift.p, (const int32_t *)p->d_ctmpl.p,
                            (const int32_t *)p->d_klo.p,
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0, tau,
@@ -1178,7 +1184,7 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
     if (p->n_items > 0)
         hipLaunchKernelGGL(refine_ascent_kernel, dim3((unsigned)p->n_items),
                            dim3(RWG), p->lds_bytes, st, (const RTmpl *)p->d_tmpl.p,
-                           (const RItem *)p->d_items.p, (const RSeg *)p->d_segs.p, order,
+                           (const RItem *)p->d_items.p, (const RSeg *)p->d_segs.p, p->lds_plane, order,
                            (const int32_t *)p->d_shift.p, (const int32_t *)p->d_ctmpl.p,
                            (const int32_t *)p->d_klo.p,
                            (const int32_t *)p->d_khi.p, cols, col_stride, (double)p->TAtom, t0,
