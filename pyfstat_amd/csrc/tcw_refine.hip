@@ -486,6 +486,10 @@ __device__ __forceinline__ void ascent_block(
     if (((int)threadIdx.x & ~63) / G >= it.ncand) return;
     const int gl = (int)threadIdx.x % G;
     const Slot sl = lane_slot<G>(it, order, shift, cand_t, tm, k_lo, k_hi);
+    // issue ahead of the register kernel's waves, which run beside this
+    // kernel to fill its idle SIMD cycles (this kernel's walks are the longer
+    // serial chains)
+    __builtin_amdgcn_s_setprio(2);
     const double t0 = sl.has ? t0v[sl.ci] : 0.0;
     const double tau_in = sl.has ? tau_iv[sl.ci] : 1.0;
     const bool live = sl.has && tau_in > 0.0;
