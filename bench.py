@@ -557,7 +557,7 @@ def run_refine(args):
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "refine_ascent_kernel<16> + refine_ascent_regs_kernel (two streams)",
+            "kernel": "refine_ascent_kernel (16 or 32 lanes a candidate) + refine_ascent_regs_kernel (two streams)",
             "achieved": achieved,
             "peak": FP64_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",

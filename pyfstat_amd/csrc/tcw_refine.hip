@@ -13,11 +13,12 @@
 //   dF/dA = (|Fb|^2 - F B)/D, dF/dB = (|Fa|^2 - F A)/D, dF/dC = 2(F C - Re(Fa Fb*))/D,
 //   dF/dFa = 2(B Fa - C Fb)/D, dF/dFb = 2(A Fb - C Fa)/D   (re and im parts),
 //   dS_c/dt0 = p_c S_c / tau,  dS_c/dtau = p_c U_c / tau^2,  U_c = sum_k w_k^{p_c} (t_k - t0) x_{c,k}.
-// All sums and the 2x2 algebra run in FP64: an FP64 add/mul/FMA issues at
-// about half the rate of an unpacked FP32 one on this part, but FP32 sums
-// would carry an eps32*cond error (up to ~6e-4 at cond = 1e4) into F and its
-// derivatives; weights come from one exp per lane and a geometric
-// recurrence along the support (t_k is a regular grid).
+// All sums and the 2x2 algebra run in FP64 (a wave's FP64 FMA issues at the
+// rate of an unpacked FP32 one here; FP32 sums would carry an eps32*cond
+// error, up to ~6e-4 at cond = 1e4, into F and its derivatives).  t_k is a
+// regular grid, so a lane's weights are w0 r^j along its bins and its sums are
+// two Horner recurrences (two FMAs a bin and column, group_sums); the lanes
+// of a candidate then add their shares with DPP exchanges.
 
 #include <hip/hip_runtime.h>
 

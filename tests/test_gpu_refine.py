@@ -1,8 +1,9 @@
 """Batched merge and exponential-window refinement on the GPU, against the
 oracle (oracle.merge_binned, oracle/refine_ref.py).
 
-Tolerances (FP64 sums on both sides; the kernel's weights come from one exp
-per lane and a geometric recurrence, the oracle's from np.exp per bin):
+Tolerances (FP64 sums on both sides; the kernel's as Horner recurrences in
+the ratio of consecutive weights, one exp a lane for the first weight, the
+oracle's from np.exp per bin):
   merge            bit-exact (float32 sums in detector order)
   2F, lnBtSG       relative 1e-8
   dF/dtau, dF/dS   relative 1e-6 of the candidate's largest component
