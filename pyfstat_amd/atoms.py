@@ -230,26 +230,34 @@ def CreateMultiFstatAtomVector(numDetectors):
     return MultiFstatAtomVector(numDetectors)
 
 
+# One SWIG atom's fields as collected by _single_to_arrays (REAL4 fields land
+# in float32 exactly; COMPLEX8 parts go through complex128 and back exactly).
+_SWIG_ATOM = np.dtype([("t", np.uint32), ("a2", np.float32), ("b2", np.float32),
+                       ("ab", np.float32), ("Fa", np.complex128), ("Fb", np.complex128)])
+
+
 def _single_to_arrays(vec):
     """(timestamps uint32 [n], columns float32 [7, n]) of one detector."""
     if isinstance(vec, FstatAtomVector):
         return vec.timestamp.copy(), vec.columns()
-    # lalpulsar SWIG object: per-atom attribute walk as tcw.py:627-638 does.
+    # lalpulsar SWIG object: the atoms' fields are only reachable one atom at a
+    # time (tcw.py:627-638 walks them into a matrix element by element); one
+    # pass collects each atom's six fields as a record, and the columns are
+    # split off the record array as whole vectors.
     n = int(vec.length)
-    ts = np.empty(n, dtype=np.uint32)
+    rec = np.fromiter(
+        ((a.timestamp, a.a2_alpha, a.b2_alpha, a.ab_alpha, a.Fa_alpha, a.Fb_alpha)
+         for a in vec.data),
+        dtype=_SWIG_ATOM, count=n)
     cols = np.empty((7, n), dtype=np.float32)
-    for i, atom in enumerate(vec.data):
-        ts[i] = atom.timestamp
-        cols[0, i] = atom.a2_alpha
-        cols[1, i] = atom.b2_alpha
-        cols[2, i] = atom.ab_alpha
-        Fa = complex(atom.Fa_alpha)
-        Fb = complex(atom.Fb_alpha)
-        cols[3, i] = Fa.real
-        cols[4, i] = Fa.imag
-        cols[5, i] = Fb.real
-        cols[6, i] = Fb.imag
-    return ts, cols
+    cols[0] = rec["a2"]
+    cols[1] = rec["b2"]
+    cols[2] = rec["ab"]
+    cols[3] = rec["Fa"].real
+    cols[4] = rec["Fa"].imag
+    cols[5] = rec["Fb"].real
+    cols[6] = rec["Fb"].imag
+    return np.ascontiguousarray(rec["t"]), cols
 
 
 class PackedAtoms:
