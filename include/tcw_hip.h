@@ -315,6 +315,53 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
                       const double *tau_lo, const double *tau_hi, int n_steps, double step0,
                       double *twoF, int32_t *n_accept, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * F-statistic atoms on the device (the step before the map path).
+ *
+ * Replaces, for one sky position and many (f, fdot) templates, the per-point
+ * host call lalpulsar.ComputeFstat(..., FSTATQ_ATOMS_PER_DET) that
+ * pyfstat/core.py:671-673 requests and core.py:1359-1365 makes once per
+ * Doppler point (lalpulsar is not part of the reference source; the sum is
+ * the resampling-free "demodulation" F-statistic restated from its
+ * definition, not lalpulsar's code).
+ *
+ * Per SFT alpha (start t_alpha, length Tsft, midpoint t_alpha + Tsft/2) and
+ * template (f, fdot) at reference time ref_time in the solar-system barycentre:
+ *   dtau  = t_alpha + Tsft/2 + delay_alpha - ref_time
+ *   phase = dtau * (f + fdot dtau / 2)                (cycles, FP64)
+ *   kappa = (f + fdot dtau) (1 + delay_rate_alpha) Tsft   (observed bin)
+ *   X     = exp(-2 pi i phase) sum_k y_k K_k,  K_k = (-1)^kf sin(pi delta) / (pi (kappa - k))
+ * over the 2*dterms bins k = kf-dterms+1 .. kf+dterms around kappa (kf =
+ * floor(kappa), delta = kappa - kf; delta = 0 takes bin kf alone), y_k the
+ * SFT bins normalised to unit noise variance (x_k / sqrt(S Tsft / 2)); bins
+ * outside the stored band count as zero.  The atoms are a2 = a^2, b2 = b^2,
+ * ab = a b, Fa = a X, Fb = b X with the SFT's antenna patterns a, b, and
+ * timestamp t_alpha, so that window sums give the transient F of the map
+ * kernels (E[2F] = 4 sum K^2 ~ 4 in Gaussian noise).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int64_t n_sft;             /* SFTs of all detectors, concatenated */
+    const uint32_t *t_start;   /* device [n_sft]: SFT start times (GPS s) */
+    const float *bins;         /* device [n_sft][n_bins][2]: normalised bins (re, im) */
+    int32_t n_bins;            /* stored bins per SFT */
+    int32_t k_min;             /* stored bin j is frequency (k_min + j) / Tsft */
+    uint32_t Tsft;             /* SFT length (s), the atoms' TAtom */
+    uint32_t reserved;
+    const double *delay;       /* device [n_sft]: SSB minus detector time at the midpoint (s) */
+    const double *delay_rate;  /* device [n_sft]: d(delay)/dt at the midpoint */
+    const float *a, *b;        /* device [n_sft]: antenna patterns at the midpoint */
+} tcw_sft_set;
+
+/* Atoms of n_tmpl templates (freq, fdot: device double [n_tmpl]) from one SFT
+ * set: template t's atom of SFT alpha is element t*n_sft + alpha of ts_out
+ * (device uint32) and of the 7 columns cols_out[c*col_stride + ...]
+ * (a2, b2, ab, Fa_re, Fa_im, Fb_re, Fb_im; col_stride >= n_tmpl*n_sft) --
+ * the layout tcw_batch_run reads, one segment per template and detector.
+ * dterms in 1..64.  Asynchronous on `stream`. */
+int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, const double *freq,
+                    const double *fdot, double ref_time, int32_t dterms, uint32_t *ts_out,
+                    float *cols_out, int64_t col_stride, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

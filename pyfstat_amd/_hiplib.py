@@ -57,6 +57,7 @@ EXPORTED_SYMBOLS = (
     "tcw_refine_forward",
     "tcw_refine_backward",
     "tcw_refine_ascent",
+    "tcw_demod_atoms",
 )
 
 TCW_REFINE_SAVED = 10
@@ -126,6 +127,22 @@ class RefineDesc(ctypes.Structure):
         ("cand_off", ctypes.c_void_p),
         ("k_lo", ctypes.c_void_p),
         ("k_hi", ctypes.c_void_p),
+    ]
+
+
+class SftSet(ctypes.Structure):
+    _fields_ = [
+        ("n_sft", ctypes.c_int64),
+        ("t_start", ctypes.c_void_p),
+        ("bins", ctypes.c_void_p),
+        ("n_bins", ctypes.c_int32),
+        ("k_min", ctypes.c_int32),
+        ("Tsft", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("delay", ctypes.c_void_p),
+        ("delay_rate", ctypes.c_void_p),
+        ("a", ctypes.c_void_p),
+        ("b", ctypes.c_void_p),
     ]
 
 
@@ -207,6 +224,9 @@ def load():
     lib.tcw_refine_ascent.restype = ctypes.c_int
     lib.tcw_refine_ascent.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, ctypes.c_int,
                                       ctypes.c_double, vp, vp, vp]
+    lib.tcw_demod_atoms.restype = ctypes.c_int
+    lib.tcw_demod_atoms.argtypes = [vp, P(SftSet), i64, vp, vp, ctypes.c_double, ctypes.c_int32,
+                                    vp, vp, i64, vp]
     _lib = lib
     return lib
 

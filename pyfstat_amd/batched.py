@@ -38,15 +38,15 @@ class BatchedTransientMaps:
     """
 
     def __init__(self, ts, cols, seg_off, seg_begin, windows, TAtom, BtSG=False,
-                 lal_strict=False, device=None, force_generic=False, rows=None):
+                 lal_strict=False, device=None, force_generic=False, rows=None,
+                 seg_tfirst=None, seg_tlast=None):
         from .tcw_fstat_map_funcs import _HipState, _ensure_handle
 
         torch = _torch()
         self.lib = _hiplib.load()
         self.handle = _ensure_handle()
         self.device = torch.device("cuda", _HipState.device if device is None else device)
-        ts = np.ascontiguousarray(ts, dtype=np.uint32)
-        cols = np.ascontiguousarray(cols, dtype=np.float32)
+        on_device = isinstance(ts, torch.Tensor)
         self.seg_off = np.ascontiguousarray(seg_off, dtype=np.int64)
         self.seg_begin = np.ascontiguousarray(seg_begin, dtype=np.int64)
         self.T = len(self.seg_begin) - 1
@@ -54,12 +54,32 @@ class BatchedTransientMaps:
             raise ValueError(f"{len(windows)} windows for {self.T} templates")
         self.windows = list(windows)
         S = len(self.seg_off) - 1
-        self.seg_tfirst = ts[self.seg_off[:-1]].copy()
-        self.seg_tlast = ts[self.seg_off[1:] - 1].copy()
-        for s in range(S):
-            seg = ts[self.seg_off[s]:self.seg_off[s + 1]]
-            if seg.size > 1 and np.any(seg[1:] < seg[:-1]):
-                raise ValueError(f"segment {s}: timestamps are not in ascending order")
+        if on_device:
+            # atoms already on the device (ts as int32 bits, cols [7, n]): the
+            # caller gives each segment's first and last timestamp and
+            # guarantees ascending timestamps within segments
+            if seg_tfirst is None or seg_tlast is None:
+                raise ValueError("device atoms need seg_tfirst and seg_tlast")
+            if ts.dtype != torch.int32 or cols.dtype != torch.float32 or cols.dim() != 2 \
+                    or cols.shape[0] != 7 or not cols.is_contiguous() or not ts.is_contiguous():
+                raise ValueError("device atoms: ts int32 [n] and contiguous cols float32 [7, n]")
+            if ts.device != self.device or cols.device != self.device:
+                raise ValueError(f"device atoms must be on {self.device}")
+            if self.seg_off[-1] > min(ts.numel(), cols.shape[1]):
+                raise ValueError("seg_off runs past the device atoms")
+            self.seg_tfirst = np.ascontiguousarray(seg_tfirst, dtype=np.uint32)
+            self.seg_tlast = np.ascontiguousarray(seg_tlast, dtype=np.uint32)
+            if self.seg_tfirst.size != S or self.seg_tlast.size != S:
+                raise ValueError("seg_tfirst / seg_tlast need one entry per segment")
+        else:
+            ts = np.ascontiguousarray(ts, dtype=np.uint32)
+            cols = np.ascontiguousarray(cols, dtype=np.float32)
+            self.seg_tfirst = ts[self.seg_off[:-1]].copy()
+            self.seg_tlast = ts[self.seg_off[1:] - 1].copy()
+            for s in range(S):
+                seg = ts[self.seg_off[s]:self.seg_off[s + 1]]
+                if seg.size > 1 and np.any(seg[1:] < seg[:-1]):
+                    raise ValueError(f"segment {s}: timestamps are not in ascending order")
         self.win_arr = (_hiplib.Window * self.T)(
             *[_hiplib.window_struct(window_fields(w)) for w in self.windows])
         self.flags = 0
@@ -89,8 +109,12 @@ class BatchedTransientMaps:
             self.rows_by_mode[int(mode)] = int(r)
         self.plan = None
         self._make_plan()
-        self.ts_d = torch.from_numpy(ts.view(np.int32)).to(self.device)
-        self.cols_d = torch.from_numpy(cols).to(self.device)
+        if on_device:
+            self.ts_d = ts
+            self.cols_d = cols
+        else:
+            self.ts_d = torch.from_numpy(ts.view(np.int32)).to(self.device)
+            self.cols_d = torch.from_numpy(cols).to(self.device)
         self.col_stride = cols.shape[1]
         self.F_d = torch.empty(int(self.F_off[-1]), dtype=torch.float32, device=self.device)
         self.stats_d = torch.empty(self.T * STATS_DTYPE.itemsize, dtype=torch.uint8,
@@ -124,6 +148,12 @@ class BatchedTransientMaps:
         return out
 
     # ------------------------------------------------------------------
+    @classmethod
+    def from_device_atoms(cls, atoms, windows, **kw):
+        """From atoms produced on the device (``demod.DeviceAtoms``): no host copy."""
+        return cls(atoms.ts_d, atoms.cols_d, atoms.seg_off, atoms.seg_begin, windows,
+                   atoms.TAtom, seg_tfirst=atoms.seg_tfirst, seg_tlast=atoms.seg_tlast, **kw)
+
     @classmethod
     def from_multi_atoms(cls, multi_list, windows, **kw):
         """From a list of MultiFstatAtomVector (one per template)."""

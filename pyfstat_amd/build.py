@@ -12,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("tcw_map.hip", "tcw_refine.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("tcw_map.hip", "tcw_refine.hip", "tcw_demod.hip")]
 INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(HERE, "_lib", "libtcw_hip.so")
 

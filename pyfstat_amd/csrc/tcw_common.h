@@ -1,5 +1,6 @@
 // tcw_common.h -- host helpers shared by the translation units of libtcw_hip.so
-// (tcw_map.hip: maps and merge; tcw_refine.hip: exponential-window refinement).
+// (tcw_map.hip: maps and merge; tcw_refine.hip: exponential-window refinement;
+// tcw_demod.hip: F-statistic atoms from SFTs).
 // Not part of the C ABI.
 #pragma once
 

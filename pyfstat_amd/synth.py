@@ -312,3 +312,91 @@ def workload_exp_refine(T=4096, n_sft=2048, n_t0=256, n_tau=256, gap_frac=0.10,
     return MultiDetectorBatch(np.concatenate(ts), np.ascontiguousarray(np.concatenate(cols, 1)),
                               np.asarray(seg_off, np.int64), np.asarray(seg_begin, np.int64),
                               windows, t_on=np.asarray(t_on), tau_inj=np.asarray(tau_inj))
+
+
+# ---------------------------------------------------------------------------
+# SFTs for the device atom producer (pyfstat_amd.demod)
+# ---------------------------------------------------------------------------
+
+SIDEREAL_DAY = 86164.0905
+YEAR = 3.15576e7
+
+
+class SyntheticSFTs:
+    """Host arrays of a synthetic SFT set (the SFTSet constructor's arguments)
+    and the injected signal's parameters."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def sft_set(self, device=None):
+        from .demod import SFTSet
+
+        return SFTSet(self.t_start, self.bins, self.k_min, self.Tsft, self.delay,
+                      self.delay_rate, self.a, self.b, self.det_off, device=device)
+
+
+def tone_sft_bins(kappa, phase, k_min, n_bins):
+    """Bins k_min .. k_min+n_bins-1 of a unit complex tone across one SFT whose
+    observed frequency is kappa bins and whose phase at the SFT midpoint is
+    `phase` cycles: (-1)^k sinc(pi (kappa - k)) exp(2 pi i phase)."""
+    k = k_min + np.arange(n_bins)
+    frac = phase - np.floor(phase)
+    return np.where(k % 2 == 1, -1.0, 1.0) * np.sinc(kappa - k) * np.exp(2j * np.pi * frac)
+
+
+def synthetic_sfts(n_det=2, n_sft=400, f_band=(100.0, 100.05), Tsft=TSFT, gap_frac=0.10,
+                   ref_time=None, inj=None, noise=True, dterms=8, seed_index=60):
+    """SFTs of n_det detectors (n_sft slots each, independent gaps) at one sky
+    position: antenna patterns a, b with sidereal modulation, an SSB delay of
+    an orbital (480 s, one year) and a rotational (0.02 s, one sidereal day)
+    term, unit-variance complex Gaussian bins (noise=True) covering every
+    template frequency in f_band with dterms bins to spare, and optionally a
+    signal: inj = dict(f, fdot, amp_a, amp_b (complex), window=(kind, t_on,
+    length)) with kind 'rect' (length = duration) or 'exp' (length = tau); the
+    signal in SFT alpha is env(t_mid) (amp_a a + amp_b b) times the tone."""
+    rng = rng_for(seed_index)
+    t_all = T_START + np.arange(n_sft, dtype=np.int64) * Tsft
+    ref = float(T_START + n_sft * Tsft // 2) if ref_time is None else float(ref_time)
+    ts, off = [], [0]
+    for _X in range(n_det):
+        keep = gap_mask(rng, n_sft, gap_frac)
+        ts.append(t_all[keep])
+        off.append(off[-1] + int(keep.sum()))
+    t_start = np.concatenate(ts).astype(np.uint32)
+    mid = t_start.astype(np.float64) + 0.5 * Tsft
+    a = np.empty(t_start.size)
+    b = np.empty(t_start.size)
+    for X in range(n_det):
+        sl = slice(off[X], off[X + 1])
+        ph = 2 * np.pi * mid[sl] / SIDEREAL_DAY + 1.3 * X
+        a[sl] = 0.45 + 0.35 * np.cos(ph) + 0.1 * np.cos(2 * ph)
+        b[sl] = 0.40 * np.sin(ph) + 0.08 * np.sin(2 * ph + 0.4)
+    w_orb, w_rot = 2 * np.pi / YEAR, 2 * np.pi / SIDEREAL_DAY
+    delay = 480.0 * np.cos(w_orb * mid + 0.7) + 0.02 * np.cos(w_rot * mid + 0.2)
+    rate = -480.0 * w_orb * np.sin(w_orb * mid + 0.7) - 0.02 * w_rot * np.sin(w_rot * mid + 0.2)
+    k_lo = int(np.floor(f_band[0] * (1 - 1.2e-4) * Tsft)) - dterms - 2
+    k_hi = int(np.ceil(f_band[1] * (1 + 1.2e-4) * Tsft)) + dterms + 2
+    n_bins = k_hi - k_lo + 1
+    bins = np.zeros((t_start.size, n_bins), dtype=np.complex128)
+    if noise:
+        bins += (rng.standard_normal(bins.shape) + 1j * rng.standard_normal(bins.shape)) / np.sqrt(2)
+    if inj is not None:
+        dtau = mid + delay - ref
+        f, fd = float(inj["f"]), float(inj.get("fdot", 0.0))
+        kind, t_on, length = inj.get("window", ("rect", T_START, n_sft * Tsft))
+        rel = mid - t_on
+        if kind == "rect":
+            env = ((rel >= 0) & (rel < length)).astype(np.float64)
+        else:
+            env = np.where((rel >= 0) & (rel <= 3 * length), np.exp(-np.clip(rel, 0, None) / length), 0.0)
+        amp = env * (complex(inj["amp_a"]) * a + complex(inj["amp_b"]) * b)
+        for al in np.nonzero(amp != 0)[0]:
+            phase = dtau[al] * (f + 0.5 * fd * dtau[al])
+            kappa = (f + fd * dtau[al]) * (1 + rate[al]) * Tsft
+            bins[al] += amp[al] * tone_sft_bins(kappa, phase, k_lo, n_bins)
+    return SyntheticSFTs(t_start=t_start, bins=bins.astype(np.complex64), k_min=k_lo, Tsft=Tsft,
+                         delay=delay, delay_rate=rate, a=a.astype(np.float32),
+                         b=b.astype(np.float32), det_off=np.asarray(off, dtype=np.int64),
+                         ref_time=ref, inj=inj, f_band=f_band)
+

@@ -65,3 +65,5 @@ def test_stats_struct_layout():
     # 4+4 + 6*8 + 8 + 2*8 + 2*8 + 4+4 = 104 bytes
     assert ctypes.sizeof(_hiplib.Stats) == 104
     assert ctypes.sizeof(_hiplib.Atoms) == 4 + 4 + 8 + 8 + 8 + 8 * 11 + 4 + 4
+    # tcw_sft_set: n_sft, two pointers, 4 x 32-bit, four pointers
+    assert ctypes.sizeof(_hiplib.SftSet) == 8 + 2 * 8 + 4 * 4 + 4 * 8

@@ -1,0 +1,128 @@
+"""CPU checks of the demodulation atom oracle (oracle/demod_ref.py) and of the
+synthetic SFT model (pyfstat_amd.synth.synthetic_sfts): closed forms, a
+brute-force DFT, the kernel's sum rule, and noise / signal statistics."""
+
+import numpy as np
+import pytest
+
+from oracle import demod_ref as dr
+from pyfstat_amd import synth
+
+
+def test_kernel_closed_form_at_bin_centre():
+    """kappa on a bin: the kernel is that bin alone, with the sign (-1)^k."""
+    for kappa, sign in ((1234.0, 1.0), (1235.0, -1.0)):
+        k0, K = dr.kernel(kappa, 8)
+        assert k0 == int(kappa) - 7
+        want = np.zeros(16)
+        want[7] = sign
+        assert np.array_equal(K, want)
+
+
+def test_tone_bins_equal_brute_force_dft():
+    """The SFT bins of a sampled complex tone (direct DFT over the SFT, unit
+    amplitude) equal (-1)^k sinc(pi (kappa - k)) e^{2 pi i phase}: the phase at
+    the SFT midpoint, the sign alternation and the sinc all pinned, up to the
+    O((kappa-k)/N) of the finite sampling."""
+    rng = np.random.default_rng(3)
+    for _ in range(4):
+        kappa = 150.0 + rng.uniform(0, 1)
+        phase = rng.uniform(0, 10)
+        n_samples = 8192
+        got = dr.tone_sft_dft(kappa, phase, 1.0, n_samples, 140, 20)
+        want = dr.tone_bins(kappa, phase, 20, 140)
+        assert np.abs(got - want).max() < 2e-3
+        syn = synth.tone_sft_bins(kappa, phase, 140, 20)
+        assert np.allclose(syn, want, rtol=1e-12, atol=1e-12)
+
+
+def test_kernel_sum_rule():
+    """sum over the 2D bins of K^2 is below 1 and tends to 1 as D grows
+    (sum over all k of sinc^2(pi (kappa - k)) = 1)."""
+    for kappa in (1000.25, 1000.5, 1000.9):
+        prev = 0.0
+        for D in (1, 2, 4, 8, 32, 64):
+            s = float(np.sum(dr.kernel(kappa, D)[1] ** 2))
+            assert prev < s < 1.0
+            prev = s
+        assert prev > 0.99
+
+
+def test_matched_filter_recovers_tone():
+    """A tone at the template's own kappa and phase demodulates to sum K^2
+    (real, positive): the kernel is matched and the phase removed."""
+    sft = synth.synthetic_sfts(n_det=1, n_sft=6, noise=False, seed_index=61,
+                               inj=dict(f=100.013, fdot=-2e-9, amp_a=1.0, amp_b=0.0))
+    a2, b2, ab, Fa, Fb = dr.demod_atoms(sft.t_start, sft.bins.astype(np.complex128), sft.k_min,
+                                        sft.Tsft, sft.delay, sft.delay_rate, sft.a, sft.b,
+                                        [100.013], [-2e-9], sft.ref_time)
+    dtau = sft.t_start + 0.5 * sft.Tsft + sft.delay - sft.ref_time
+    for al in range(sft.t_start.size):
+        kappa = (100.013 - 2e-9 * dtau[al]) * (1 + sft.delay_rate[al]) * sft.Tsft
+        g = np.sum(dr.kernel(kappa, 8)[1] ** 2)
+        X = Fa[0, al] / sft.a[al]
+        want = sft.a[al] * g  # the injected amplitude a (amp_a = 1)
+        assert abs(X - want) < 1e-5 * abs(want) + 1e-6
+
+
+def test_noise_mean_twoF():
+    """Gaussian unit-variance bins: E[2F] = 4 sum K^2 (~3.9 for D = 8), over
+    many templates and two noise realisations."""
+    vals, gs = [], []
+    for seed in (62, 63):
+        sft = synth.synthetic_sfts(n_det=2, n_sft=60, noise=True, seed_index=seed)
+        freqs = np.linspace(100.0, 100.05, 160)
+        a2, b2, ab, Fa, Fb = dr.demod_atoms(sft.t_start, sft.bins.astype(np.complex128),
+                                            sft.k_min, sft.Tsft, sft.delay, sft.delay_rate,
+                                            sft.a, sft.b, freqs, np.zeros_like(freqs),
+                                            sft.ref_time)
+        vals.append(dr.twoF_all_data(a2, b2, ab, Fa, Fb))
+        gs.append(np.mean([np.sum(dr.kernel(f * sft.Tsft + 0.37, 8)[1] ** 2) for f in freqs]))
+    v = np.concatenate(vals)
+    want = 4 * np.mean(gs)
+    # templates 1/T_obs apart are nearly independent: the mean of 320 values
+    # of a chi^2_4 / ... has standard error ~ 2.8 / sqrt(320) ~ 0.16
+    assert abs(v.mean() - want) < 0.5, (v.mean(), want)
+
+
+def test_injected_signal_recovered():
+    """A noise-free signal gives 2F = 2 z^H M z (sum K^2)^2 at its template,
+    with M = [[A, C], [C, B]] over the SFTs and z = (amp_a, amp_b), and less at
+    a template one frequency bin of the observation away."""
+    inj = dict(f=100.021, fdot=0.0, amp_a=0.3 + 0.1j, amp_b=-0.2j)
+    sft = synth.synthetic_sfts(n_det=2, n_sft=80, noise=False, seed_index=64, inj=inj)
+    T_obs = 80 * sft.Tsft
+    freqs = [inj["f"], inj["f"] + 1.0 / T_obs]
+    a2, b2, ab, Fa, Fb = dr.demod_atoms(sft.t_start, sft.bins.astype(np.complex128), sft.k_min,
+                                        sft.Tsft, sft.delay, sft.delay_rate, sft.a, sft.b,
+                                        freqs, [0.0, 0.0], sft.ref_time)
+    twoF = dr.twoF_all_data(a2, b2, ab, Fa, Fb)
+    a = sft.a.astype(np.float64)
+    b = sft.b.astype(np.float64)
+    dtau = sft.t_start + 0.5 * sft.Tsft + sft.delay - sft.ref_time
+    g = np.array([np.sum(dr.kernel(inj["f"] * (1 + r) * sft.Tsft, 8)[1] ** 2)
+                  for r in sft.delay_rate])
+    del dtau
+    # with g varying slightly per SFT, the exact noise-free value is
+    # F = (B|Fa|^2 + A|Fb|^2 - 2C Re(Fa Fb*)) / D with Fa = sum a g s, Fb = sum b g s
+    s = inj["amp_a"] * a + inj["amp_b"] * b
+    Fa_w, Fb_w = np.sum(a * g * s), np.sum(b * g * s)
+    A, B, C = np.sum(a * a), np.sum(b * b), np.sum(a * b)
+    want = 2 * (B * abs(Fa_w) ** 2 + A * abs(Fb_w) ** 2
+                - 2 * C * np.real(Fa_w * np.conj(Fb_w))) / (A * B - C * C)
+    assert abs(twoF[0] - want) < 1e-6 * want
+    assert twoF[1] < 0.2 * twoF[0]
+
+
+@pytest.mark.parametrize("kind", ["rect", "exp"])
+def test_transient_envelope_in_synthetic_sfts(kind):
+    """The injected envelope: SFTs outside the transient carry no signal."""
+    t_on = synth.T_START + 20 * synth.TSFT
+    inj = dict(f=100.01, fdot=0.0, amp_a=1.0, amp_b=0.0, window=(kind, t_on, 10 * synth.TSFT))
+    sft = synth.synthetic_sfts(n_det=1, n_sft=60, gap_frac=0.0, noise=False, seed_index=65,
+                               inj=inj)
+    power = np.abs(sft.bins).sum(axis=1)
+    mid = sft.t_start + 0.5 * sft.Tsft
+    before = mid < t_on
+    assert np.all(power[before] == 0)
+    assert np.all(power[~before][:5] > 0)
