@@ -43,7 +43,8 @@ def kernel(kappa, dterms):
 def demod_atoms(t_start, bins, k_min, Tsft, delay, delay_rate, a, b, freq, fdot, ref_time,
                 dterms=8):
     """Atoms [n_tmpl, n_sft]: returns (a2, b2, ab, Fa complex, Fb complex) as
-    float64 / complex128 arrays.  bins: complex [n_sft, n_bins]."""
+    float64 / complex128 arrays.  bins: complex [n_sft, n_bins].  Vectorised
+    over the SFTs; the same sum as kernel() per SFT."""
     t_start = np.asarray(t_start, dtype=np.float64)
     n_sft, n_bins = bins.shape
     freq = np.atleast_1d(np.asarray(freq, dtype=np.float64))
@@ -51,16 +52,25 @@ def demod_atoms(t_start, bins, k_min, Tsft, delay, delay_rate, a, b, freq, fdot,
     X = np.zeros((freq.size, n_sft), dtype=np.complex128)
     dtau = t_start + 0.5 * Tsft + np.asarray(delay, dtype=np.float64) - ref_time
     dop = (1.0 + np.asarray(delay_rate, dtype=np.float64)) * Tsft
+    i = np.arange(2 * dterms)
+    rows = np.arange(n_sft)[:, None]
     for t in range(freq.size):
-        for al in range(n_sft):
-            phase = dtau[al] * (freq[t] + 0.5 * fdot[t] * dtau[al])
-            kappa = (freq[t] + fdot[t] * dtau[al]) * dop[al]
-            k0, K = kernel(kappa, dterms)
-            j = k0 - k_min + np.arange(2 * dterms)
-            ok = (j >= 0) & (j < n_bins)
-            s = np.sum(bins[al, j[ok]] * K[ok])
-            frac = phase - np.floor(phase)
-            X[t, al] = s * np.exp(-2j * np.pi * frac)
+        phase = dtau * (freq[t] + 0.5 * fdot[t] * dtau)
+        kappa = (freq[t] + fdot[t] * dtau) * dop
+        kf = np.floor(kappa)
+        delta = kappa - kf
+        k = kf[:, None] - dterms + 1 + i[None, :]
+        sign = np.where(kf.astype(np.int64) % 2 == 1, -1.0, 1.0)[:, None]
+        on_bin = (delta == 0.0)[:, None]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            K = np.where(on_bin, np.where(k == kf[:, None], sign, 0.0),
+                         sign * np.sin(np.pi * delta)[:, None] / (np.pi * (kappa[:, None] - k)))
+        j = k.astype(np.int64) - k_min
+        ok = (j >= 0) & (j < n_bins)
+        y = np.where(ok, bins[rows, np.clip(j, 0, n_bins - 1)], 0.0)
+        s = np.sum(y * K, axis=1)
+        frac = phase - np.floor(phase)
+        X[t] = s * np.exp(-2j * np.pi * frac)
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     a2 = np.broadcast_to(a * a, X.shape)

@@ -123,6 +123,22 @@ class DeviceAtoms:
         return (self.ts_d.cpu().numpy().view(np.uint32), self.cols_d.cpu().numpy())
 
 
+def demod_atoms_into(sfts, freq_d, fdot_d, ref_time, ts_d, cols_d, dterms=8, stream=None):
+    """tcw_demod_atoms on device arrays the caller owns: freq_d, fdot_d
+    float64 [T]; ts_d int32 [>= T*n_sft], cols_d float32 [7, >= T*n_sft]."""
+    torch = _torch()
+    T = int(freq_d.numel())
+    if int(fdot_d.numel()) != T:
+        raise ValueError("freq and fdot must have the same length")
+    if ts_d.numel() < T * sfts.n_sft or cols_d.dim() != 2 or cols_d.shape[0] != NC:
+        raise ValueError("output buffers too small for T * n_sft atoms")
+    st = torch.cuda.current_stream(sfts.device) if stream is None else stream
+    _hiplib.check(sfts.lib.tcw_demod_atoms(
+        sfts.handle, ctypes.byref(sfts.desc), T, freq_d.data_ptr(), fdot_d.data_ptr(),
+        float(ref_time), int(dterms), ts_d.data_ptr(), cols_d.data_ptr(), cols_d.shape[1],
+        ctypes.c_void_p(st.cuda_stream)), "tcw_demod_atoms")
+
+
 def demod_atoms(sfts, freq, fdot, ref_time, dterms=8, stream=None):
     """Atoms of the templates (freq[i], fdot[i]) at reference time ref_time
     (SSB GPS s) from ``sfts`` (an :class:`SFTSet`), on the device."""
@@ -137,11 +153,8 @@ def demod_atoms(sfts, freq, fdot, ref_time, dterms=8, stream=None):
     total = T * sfts.n_sft
     ts_d = torch.empty(max(total, 1), dtype=torch.int32, device=sfts.device)
     cols_d = torch.empty((NC, max(total, 1)), dtype=torch.float32, device=sfts.device)
-    st = torch.cuda.current_stream(sfts.device) if stream is None else stream
-    _hiplib.check(sfts.lib.tcw_demod_atoms(
-        sfts.handle, ctypes.byref(sfts.desc), T, f.data_ptr(), fd.data_ptr(), float(ref_time),
-        int(dterms), ts_d.data_ptr(), cols_d.data_ptr(), cols_d.shape[1],
-        ctypes.c_void_p(st.cuda_stream)), "tcw_demod_atoms")
+    if T > 0:
+        demod_atoms_into(sfts, f, fd, ref_time, ts_d, cols_d, dterms=dterms, stream=stream)
     out = DeviceAtoms(sfts, ts_d, cols_d, T)
     out._keep = (f, fd)  # inputs stay alive until the queued kernel has run
     return out
