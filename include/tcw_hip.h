@@ -332,7 +332,7 @@ int tcw_refine_ascent(tcw_refine_plan *p, const float *cols, int64_t col_stride,
  *   kappa = (f + fdot dtau) (1 + delay_rate_alpha) Tsft   (observed bin)
  *   X     = exp(-2 pi i phase) sum_k y_k K_k,  K_k = (-1)^kf sin(pi delta) / (pi (kappa - k))
  * over the 2*dterms bins k = kf-dterms+1 .. kf+dterms around kappa (kf =
- * floor(kappa), delta = kappa - kf; delta = 0 takes bin kf alone), y_k the
+ * floor(kappa), delta = kappa - kf; delta below 1e-30 takes bin kf alone), y_k the
  * SFT bins normalised to unit noise variance (x_k / sqrt(S Tsft / 2)); bins
  * outside the stored band count as zero.  The atoms are a2 = a^2, b2 = b^2,
  * ab = a b, Fa = a X, Fb = b X with the SFT's antenna patterns a, b, and
