@@ -656,10 +656,10 @@ struct Reducer {
                         rr = r;
                         cc = c;
                     }
+            // a lane's columns are evenly spaced (1 or 64 apart): arithmetic,
+            // not a selection the compiler would turn into a private array
             int col = colidx[0];
-#pragma unroll
-            for (int c = 1; c < NCOLS; c++)
-                if (cc == c) col = colidx[c];
+            if constexpr (NCOLS > 1) col += cc * (colidx[1] - colidx[0]);
             argmax_merge(best, bidx, lm, (m0 + rr) * N_tau + col);
         }
         if (!BTSG) return;
