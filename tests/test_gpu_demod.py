@@ -101,6 +101,26 @@ def test_band_edge_and_bin_centre(hip_ready):
             assert err < ATOL_REL, (t, c, err)
 
 
+@pytest.mark.parametrize("T", [2, 5, 40])
+def test_template_tile_sizes(hip_ready, T):
+    """Every tile shape the host picks (1, 8 and 64 template slots a
+    workgroup for 2, 5 and 40 templates) with dterms 8 (unrolled) and 5
+    (runtime run lengths), against the oracle; 40 templates leave a partial
+    64-template tile and 260 SFTs a partial 64-SFT tile."""
+    from pyfstat_amd.demod import demod_atoms
+
+    sft = synth.synthetic_sfts(n_det=2, n_sft=130, gap_frac=0.0, seed_index=75)
+    sfts = sft.sft_set()
+    rng = np.random.default_rng(75 + T)
+    freqs = rng.uniform(100.0, 100.05, T)
+    fdots = rng.uniform(-5e-9, 5e-9, T)
+    for dterms in (8, 5):
+        print(f"case: {T} templates, dterms {dterms}", flush=True)
+        at = demod_atoms(sfts, freqs, fdots, sft.ref_time, dterms=dterms)
+        ts, cols = at.host()
+        _check_atoms(ts, cols, sft, freqs, fdots, dterms)
+
+
 def test_device_atoms_feed_batched_maps(hip_ready):
     """SFTs -> device atoms -> batched rect maps without a host copy: the maps
     equal those of the same atoms uploaded from the host, match the oracle's
