@@ -15,6 +15,11 @@ Two independent restatements of the reference algorithm live here:
 * ``twin.py``: a plain-Python loop restatement of the same steps for tiny
   inputs, used to cross-check the C oracle.
 
+``tcw_oracle.c`` also restates, for timing, the refinement evaluation of
+``refine_ref.py`` and the demodulation atom sum of ``demod_ref.py``
+(``demod_atoms`` below; checked against the numpy version in
+tests/test_demod.py).
+
 How each function is pinned (closed forms, invariants, brute force, library
 routines) is listed in tests/test_oracle.py.  Parity unpinned: lalpulsar's
 lookup-table exp (XLALFastNegExp) and its REAL4 accumulation order, which only
@@ -140,6 +145,10 @@ def _load():
                                       P(ctypes.c_int64), P(ctypes.c_double),
                                       P(ctypes.c_int64), P(ctypes.c_int64),
                                       P(ctypes.c_double)]
+        lib.orc_demod_atoms.restype = ctypes.c_int
+        lib.orc_demod_atoms.argtypes = [ctypes.c_int64, vp, vp, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_double, vp, vp, vp, vp, ctypes.c_int64, vp,
+                                        vp, ctypes.c_double, ctypes.c_int32, vp, vp]
         _lib = lib
     return _lib
 
@@ -332,3 +341,37 @@ def refine_eval_batch(cols, m_off, t_data, TA, cand_tmpl, k_lo, k_hi, t0, tau):
     if rc:
         raise ValueError("oracle refine batch failed")
     return F, dt0, dtau, g
+
+
+def demod_atoms(t_start, bins, k_min, Tsft, delay, delay_rate, a, b, freq, fdot, ref_time,
+                dterms=8):
+    """C restatement of demod_ref.demod_atoms (FP64, OpenMP over templates and
+    SFTs); the same return values: (a2, b2, ab, Fa complex, Fb complex), each
+    [n_tmpl, n_sft]."""
+    t_start = np.ascontiguousarray(t_start, dtype=np.uint32)
+    n = t_start.size
+    bins = np.asarray(bins)
+    pair = np.empty((n, bins.shape[1], 2), dtype=np.float32)
+    pair[..., 0] = bins.real
+    pair[..., 1] = bins.imag
+    delay = np.ascontiguousarray(delay, dtype=np.float64)
+    rate = np.ascontiguousarray(delay_rate, dtype=np.float64)
+    a32 = np.ascontiguousarray(a, dtype=np.float32)
+    b32 = np.ascontiguousarray(b, dtype=np.float32)
+    freq = np.ascontiguousarray(np.atleast_1d(freq), dtype=np.float64)
+    fdot = np.ascontiguousarray(np.atleast_1d(fdot), dtype=np.float64)
+    T = freq.size
+    Fa = np.empty((T, n, 2))
+    Fb = np.empty((T, n, 2))
+    rc = _load().orc_demod_atoms(n, t_start.ctypes.data, pair.ctypes.data, pair.shape[1],
+                                 int(k_min), float(Tsft), delay.ctypes.data, rate.ctypes.data,
+                                 a32.ctypes.data, b32.ctypes.data, T, freq.ctypes.data,
+                                 fdot.ctypes.data, float(ref_time), int(dterms), Fa.ctypes.data,
+                                 Fb.ctypes.data)
+    if rc:
+        raise ValueError("oracle demod failed")
+    ad, bd = a32.astype(np.float64), b32.astype(np.float64)
+    shape = (T, n)
+    return (np.broadcast_to(ad * ad, shape), np.broadcast_to(bd * bd, shape),
+            np.broadcast_to(ad * bd, shape), Fa[..., 0] + 1j * Fa[..., 1],
+            Fb[..., 0] + 1j * Fb[..., 1])

@@ -405,3 +405,64 @@ int orc_refine_eval_batch(int64_t n, const float *cols, int64_t col_stride,
     }
     return 0;
 }
+
+/* F-statistic atoms of the resampling-free demodulation sum (the formula of
+ * oracle/demod_ref.py's demod_atoms, restated in C for timing): for template
+ * t and SFT a,
+ *   dtau  = t_start + Tsft/2 + delay - ref_time,
+ *   phase = dtau (f + fdot dtau / 2)   (cycles),
+ *   kappa = (f + fdot dtau) (1 + delay_rate) Tsft,
+ *   X     = exp(-2 pi i frac(phase)) sum_k y_k (-1)^kf sin(pi delta) / (pi (kappa - k))
+ * over k = kf-D+1 .. kf+D (kf = floor(kappa), delta = kappa - kf; delta = 0:
+ * bin kf alone), out-of-band bins zero.  FP64; bins are float32 (re, im)
+ * pairs [n_sft][n_bins][2].  Writes Fa = a X and Fb = b X as (re, im) pairs
+ * [n_tmpl][n_sft][2].  OpenMP over templates and SFTs. */
+int orc_demod_atoms(int64_t n_sft, const uint32_t *t_start, const float *bins, int32_t n_bins,
+                    int32_t k_min, double Tsft, const double *delay, const double *delay_rate,
+                    const float *a, const float *b, int64_t n_tmpl, const double *freq,
+                    const double *fdot, double ref_time, int32_t dterms, double *Fa,
+                    double *Fb)
+{
+    if (dterms < 1 || n_bins < 1) return 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t t = 0; t < n_tmpl; t++) {
+        for (int64_t s = 0; s < n_sft; s++) {
+            const double dtau = (double)t_start[s] + 0.5 * Tsft + delay[s] - ref_time;
+            const double phase = dtau * (freq[t] + 0.5 * fdot[t] * dtau);
+            const double dop = (1.0 + delay_rate[s]) * Tsft;
+            const double kappa = (freq[t] + fdot[t] * dtau) * dop;
+            const double kf = floor(kappa);
+            const double delta = kappa - kf;
+            const int64_t kfi = (int64_t)kf;
+            const double sign = (kfi % 2 != 0) ? -1.0 : 1.0;
+            const float *row = bins + s * (int64_t)n_bins * 2;
+            double sr = 0.0, si = 0.0;
+            if (delta == 0.0) {
+                const int64_t j = kfi - k_min;
+                if (j >= 0 && j < n_bins) {
+                    sr = sign * (double)row[2 * j];
+                    si = sign * (double)row[2 * j + 1];
+                }
+            } else {
+                const double amp = sign * sin(M_PI * delta);
+                for (int i = 0; i < 2 * dterms; i++) {
+                    const int64_t k = kfi - dterms + 1 + i;
+                    const int64_t j = k - k_min;
+                    if (j < 0 || j >= n_bins) continue;
+                    const double K = amp / (M_PI * (kappa - (double)k));
+                    sr += K * (double)row[2 * j];
+                    si += K * (double)row[2 * j + 1];
+                }
+            }
+            const double frac = phase - floor(phase);
+            const double c = cos(2.0 * M_PI * frac), sn = sin(2.0 * M_PI * frac);
+            const double xr = sr * c + si * sn, xi = si * c - sr * sn;
+            const int64_t o = (t * n_sft + s) * 2;
+            Fa[o] = (double)a[s] * xr;
+            Fa[o + 1] = (double)a[s] * xi;
+            Fb[o] = (double)b[s] * xr;
+            Fb[o + 1] = (double)b[s] * xi;
+        }
+    }
+    return 0;
+}

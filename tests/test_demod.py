@@ -5,6 +5,7 @@ brute-force DFT, the kernel's sum rule, and noise / signal statistics."""
 import numpy as np
 import pytest
 
+import oracle
 from oracle import demod_ref as dr
 from pyfstat_amd import synth
 
@@ -126,3 +127,21 @@ def test_transient_envelope_in_synthetic_sfts(kind):
     before = mid < t_on
     assert np.all(power[before] == 0)
     assert np.all(power[~before][:5] > 0)
+
+
+@pytest.mark.parametrize("dterms", [8, 3])
+def test_c_restatement_equals_numpy_oracle(dterms):
+    """The C restatement (oracle/tcw_oracle.c, the CPU baseline) computes the
+    same FP64 sums as demod_ref.demod_atoms, up to summation order."""
+    inj = dict(f=100.0213, fdot=-3e-9, amp_a=0.4 + 0.2j, amp_b=0.1 - 0.3j)
+    sft = synth.synthetic_sfts(n_det=2, n_sft=90, seed_index=66, inj=inj)
+    freqs = np.array([inj["f"], inj["f"] + 2e-5, 100.0005, 100.049])
+    fdots = np.array([inj["fdot"], 0.0, 1e-9, -5e-9])
+    want = dr.demod_atoms(sft.t_start, sft.bins.astype(np.complex128), sft.k_min, sft.Tsft,
+                          sft.delay, sft.delay_rate, sft.a, sft.b, freqs, fdots, sft.ref_time,
+                          dterms=dterms)
+    got = oracle.demod_atoms(sft.t_start, sft.bins, sft.k_min, sft.Tsft, sft.delay,
+                             sft.delay_rate, sft.a, sft.b, freqs, fdots, sft.ref_time,
+                             dterms=dterms)
+    for w, g in zip(want, got):
+        assert np.abs(w - g).max() <= 1e-13 * np.abs(w).max()
