@@ -300,6 +300,9 @@ int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, cons
     if (!h || !sfts) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: null argument");
     if (sfts->n_sft < 1) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: no SFTs");
     if (sfts->n_bins < 1) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: no SFT bins");
+    // bin indices are int32 in the kernel: the band must lie in 0 .. 2^31 - 1
+    if (sfts->k_min < 0 || (int64_t)sfts->k_min + sfts->n_bins > INT32_MAX)
+        return fail(TCW_ERR_INVALID, "SFT band must lie in bins 0 .. 2^31-1");
     if (sfts->Tsft < 2) return fail(TCW_ERR_INVALID, "Tsft must be at least 2 seconds.");
     if (dterms < 1 || dterms > DTERMS_MAX)
         return fail(TCW_ERR_INVALID, "dterms must be in 1.." + std::to_string(DTERMS_MAX));

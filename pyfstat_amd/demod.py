@@ -69,6 +69,8 @@ class SFTSet:
                 raise ValueError(f"detector {X}: SFT start times are not ascending")
         if int(Tsft) < 2:
             raise ValueError("Tsft must be at least 2 seconds.")
+        if int(k_min) < 0 or int(k_min) + bins.shape[1] > 2**31 - 1:
+            raise ValueError("SFT band must lie in bins 0 .. 2^31-1")
         self.t_start = t
         self.n_sft = int(t.size)
         self.n_bins = int(bins.shape[1])

@@ -186,3 +186,11 @@ def test_validation(hip_ready):
     with pytest.raises(ValueError, match="Tsft"):
         SFTSet(sft.t_start, sft.bins, sft.k_min, 1, sft.delay, sft.delay_rate, sft.a, sft.b,
                sft.det_off)
+    with pytest.raises(ValueError, match="band"):
+        SFTSet(sft.t_start, sft.bins, -1, sft.Tsft, sft.delay, sft.delay_rate, sft.a, sft.b,
+               sft.det_off)
+    # the C ABI checks the band itself (a descriptor filled by another caller)
+    sfts.desc.k_min = 2**31 - 10
+    with pytest.raises(ValueError, match="band"):
+        demod_atoms(sfts, [100.01], [0.0], sft.ref_time)
+    sfts.desc.k_min = sft.k_min
