@@ -12,17 +12,19 @@
 // the Doppler-shifted frequency, and the signal phase at the SFT midpoint.
 //
 // Tiles of TT templates x 64 SFTs per workgroup, two phases:
-//   compute  lanes are templates of one SFT (TT = 64: the SFT's timing, Doppler
-//            factor and antenna patterns are wave-uniform scalar loads, and the
+//   compute  lanes are templates of one SFT (TT = 64; the SFTs' dtau and
+//            Doppler factors are formed once per workgroup into LDS), so the
 //            lanes read the same few SFT bins, since neighbouring templates
-//            fall in the same bins); the atom's complex X goes to LDS.
+//            fall in the same bins; the atom's complex X goes to LDS.
 //   write    lanes are SFTs of one template: each of the 8 output columns is
-//            written as 64 consecutive 4-byte values per wave (full lines).
+//            written as 64 consecutive 4-byte values per wave (full lines,
+//            nontemporal stores).
 // The phase is FP64 (f * dtau reaches 1e10 cycles); the kernel weights and
 // the complex sum are float32 (the atoms are float32), summed as one rational
 // function per run of up to 8 bins (one reciprocal per run instead of one
-// per bin).  Bound: the 32 bytes written per atom (HBM); the bins read are a
-// few MB shared by every template and stay in the caches.
+// per bin); sin(pi delta) and the rotation are Taylor polynomials on reduced
+// arguments.  Bound: the 32 bytes written per atom (HBM); the bins read are
+// a few MB shared by every template and stay in the caches.
 
 #include <hip/hip_runtime.h>
 
