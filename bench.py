@@ -76,9 +76,10 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # What bounds the map kernel instead of HBM (rocprofv3 counters of the same
 # kernel, summarised in the profile files named here).
 LIMITER = {
-    "rect_batched": "VALU issue: 47.25 VALU instructions per cell (22 FP64 add/mul/FMA), "
-                    "SQ_ACTIVE_INST_VALU ~99% of SIMD cycles at the ~1.9 GHz the card holds under "
-                    "this load; profiles/rect_fast_kernel.rect_batched.md",
+    "rect_batched": "VALU issue: 44-47 VALU instructions per cell (22 FP64 add/mul/FMA; 40.4 on "
+                    "maps of whole 256-column blocks: 8.6% of the lane-cells here are lanes past "
+                    "N_tau), SQ_ACTIVE_INST_VALU ~99% of SIMD cycles at the ~1.9 GHz the card "
+                    "holds under this load; profiles/rect_fast_kernel.rect_batched.md",
     "exp_shared": "latency at two waves per SIMD (one row chunk per column: 2048 waves): "
                   "~56 VALU instructions per cell at ~59% VALU busy, FP64 dependency chains and "
                   "LDS broadcasts of the entering atoms; profiles/exp_fast_kernel.exp_shared.md",
