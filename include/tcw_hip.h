@@ -362,6 +362,20 @@ int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, cons
                     const double *fdot, double ref_time, int32_t dterms, uint32_t *ts_out,
                     float *cols_out, int64_t col_stride, void *stream);
 
+/* tcw_demod_atoms for n_sky sky positions in one launch (the reference calls
+ * lalpulsar.ComputeFstat once per Doppler point, pyfstat/core.py:1359-1365;
+ * here the (f, fdot) grid of every sky point at once): the SFT bins and start
+ * times come from `sfts`, the per-SFT delay, delay_rate, a and b of sky point
+ * s from the device arrays delay[s*n_sft + alpha] etc. (sfts' own delay/a/b
+ * pointers are not read).  Sky s, template t, SFT alpha is element
+ * (s*n_tmpl + t)*n_sft + alpha of ts_out and of each column
+ * (col_stride >= n_sky*n_tmpl*n_sft); n_sky in 1..65535. */
+int tcw_demod_atoms_sky(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_sky,
+                        const double *delay, const double *delay_rate, const float *a,
+                        const float *b, int64_t n_tmpl, const double *freq, const double *fdot,
+                        double ref_time, int32_t dterms, uint32_t *ts_out, float *cols_out,
+                        int64_t col_stride, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

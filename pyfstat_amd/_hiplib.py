@@ -58,6 +58,7 @@ EXPORTED_SYMBOLS = (
     "tcw_refine_backward",
     "tcw_refine_ascent",
     "tcw_demod_atoms",
+    "tcw_demod_atoms_sky",
 )
 
 TCW_REFINE_SAVED = 10
@@ -227,6 +228,9 @@ def load():
     lib.tcw_demod_atoms.restype = ctypes.c_int
     lib.tcw_demod_atoms.argtypes = [vp, P(SftSet), i64, vp, vp, ctypes.c_double, ctypes.c_int32,
                                     vp, vp, i64, vp]
+    lib.tcw_demod_atoms_sky.restype = ctypes.c_int
+    lib.tcw_demod_atoms_sky.argtypes = [vp, P(SftSet), i64, vp, vp, vp, vp, i64, vp, vp,
+                                        ctypes.c_double, ctypes.c_int32, vp, vp, i64, vp]
     _lib = lib
     return lib
 

@@ -210,7 +210,9 @@ __device__ __forceinline__ float2 demod_one(const float2 *__restrict__ row, int 
 }
 
 // TT templates x SFT_TILE SFTs per workgroup; grid x over SFT tiles, grid y
-// strides over template tiles.  DT: dterms at compile time, or 0 (runtime).
+// strides over template tiles, grid z is the sky point (its per-SFT delay,
+// rate and antenna patterns, and its block of n_tmpl templates in the
+// output).  DT: dterms at compile time, or 0 (runtime).
 template <int TT, int DT>
 __global__ __launch_bounds__(DEMOD_WG) void demod_atoms_kernel(
     int64_t n_sft, int64_t n_tmpl, const uint32_t *__restrict__ t_start,
@@ -227,6 +229,15 @@ __global__ __launch_bounds__(DEMOD_WG) void demod_atoms_kernel(
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t a0 = (int64_t)blockIdx.x * SFT_TILE;
+    {  // this workgroup's sky point
+        const int64_t sky = blockIdx.z;
+        delay += sky * n_sft;
+        delay_rate += sky * n_sft;
+        pa += sky * n_sft;
+        pb += sky * n_sft;
+        ts_out += sky * n_tmpl * n_sft;
+        cols += sky * n_tmpl * n_sft;
+    }
     // write phase: this lane's SFT (the wave's template rows are uniform, so
     // every store is a uniform row pointer plus the lane's 32-bit offset)
     const int64_t wa = a0 + lane;
@@ -292,13 +303,12 @@ __global__ __launch_bounds__(DEMOD_WG) void demod_atoms_kernel(
     }
 }
 
-}  // namespace
-
-extern "C" {
-
-int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, const double *freq,
-                    const double *fdot, double ref_time, int32_t dterms, uint32_t *ts_out,
-                    float *cols_out, int64_t col_stride, void *stream) {
+// Both entry points: argument checks, then the launch (n_sky sky points on
+// grid z; delay .. b are [n_sky][n_sft]).
+int demod_launch(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_sky, const double *delay,
+                 const double *delay_rate, const float *a, const float *b, int64_t n_tmpl,
+                 const double *freq, const double *fdot, double ref_time, int32_t dterms,
+                 uint32_t *ts_out, float *cols_out, int64_t col_stride, void *stream) {
     if (!h || !sfts) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: null argument");
     if (sfts->n_sft < 1) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: no SFTs");
     if (sfts->n_bins < 1) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: no SFT bins");
@@ -308,13 +318,15 @@ int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, cons
     if (sfts->Tsft < 2) return fail(TCW_ERR_INVALID, "Tsft must be at least 2 seconds.");
     if (dterms < 1 || dterms > DTERMS_MAX)
         return fail(TCW_ERR_INVALID, "dterms must be in 1.." + std::to_string(DTERMS_MAX));
+    if (n_sky < 1 || n_sky > (int64_t)DEMOD_GRID_Y)
+        return fail(TCW_ERR_INVALID, "sky points must number 1.." + std::to_string(DEMOD_GRID_Y));
     if (n_tmpl < 0) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: negative template count");
     if (n_tmpl == 0) return TCW_OK;
-    if (!sfts->t_start || !sfts->bins || !sfts->delay || !sfts->delay_rate || !sfts->a ||
-        !sfts->b || !freq || !fdot || !ts_out || !cols_out)
+    if (!sfts->t_start || !sfts->bins || !delay || !delay_rate || !a || !b || !freq || !fdot ||
+        !ts_out || !cols_out)
         return fail(TCW_ERR_INVALID, "tcw_demod_atoms: null array");
-    if (n_tmpl > INT64_MAX / sfts->n_sft || col_stride < n_tmpl * sfts->n_sft)
-        return fail(TCW_ERR_INVALID, "col_stride smaller than n_tmpl * n_sft");
+    if (n_tmpl > INT64_MAX / sfts->n_sft / n_sky || col_stride < n_sky * n_tmpl * sfts->n_sft)
+        return fail(TCW_ERR_INVALID, "col_stride smaller than n_sky * n_tmpl * n_sft");
     if (!std::isfinite(ref_time)) return fail(TCW_ERR_INVALID, "ref_time must be finite");
     const int rc = set_device(tcwc::handle_device(h));
     if (rc) return rc;
@@ -324,15 +336,14 @@ int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, cons
     // SFT when there are only a few templates
     const int tt = n_tmpl >= 32 ? 64 : n_tmpl >= 4 ? 8 : 1;
     const unsigned gy = (unsigned)std::min<int64_t>((n_tmpl + tt - 1) / tt, DEMOD_GRID_Y);
-    const dim3 grid((unsigned)bx, gy);
+    const dim3 grid((unsigned)bx, gy, (unsigned)n_sky);
     const auto *bins = (const float2 *)sfts->bins;
     const double Tsft = (double)sfts->Tsft;
 #define TCW_DEMOD_LAUNCH(TT, DT)                                                               \
     hipLaunchKernelGGL((demod_atoms_kernel<TT, DT>), grid, dim3(DEMOD_WG), 0,                  \
                        (hipStream_t)stream, sfts->n_sft, n_tmpl, sfts->t_start, bins,          \
-                       sfts->n_bins, sfts->k_min, Tsft, sfts->delay, sfts->delay_rate,         \
-                       sfts->a, sfts->b, freq, fdot, ref_time, dterms, ts_out, cols_out,       \
-                       col_stride)
+                       sfts->n_bins, sfts->k_min, Tsft, delay, delay_rate, a, b, freq, fdot,   \
+                       ref_time, dterms, ts_out, cols_out, col_stride)
     // dterms 8 (lalpulsar's default) unrolled at compile time
     if (tt == 64) {
         if (dterms == 8) TCW_DEMOD_LAUNCH(64, 8); else TCW_DEMOD_LAUNCH(64, 0);
@@ -344,6 +355,27 @@ int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, cons
 #undef TCW_DEMOD_LAUNCH
     TCW_HIP_TRY(hipGetLastError());
     return TCW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcw_demod_atoms(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_tmpl, const double *freq,
+                    const double *fdot, double ref_time, int32_t dterms, uint32_t *ts_out,
+                    float *cols_out, int64_t col_stride, void *stream) {
+    if (!h || !sfts) return fail(TCW_ERR_INVALID, "tcw_demod_atoms: null argument");
+    return demod_launch(h, sfts, 1, sfts->delay, sfts->delay_rate, sfts->a, sfts->b, n_tmpl,
+                        freq, fdot, ref_time, dterms, ts_out, cols_out, col_stride, stream);
+}
+
+int tcw_demod_atoms_sky(tcw_handle *h, const tcw_sft_set *sfts, int64_t n_sky,
+                        const double *delay, const double *delay_rate, const float *a,
+                        const float *b, int64_t n_tmpl, const double *freq, const double *fdot,
+                        double ref_time, int32_t dterms, uint32_t *ts_out, float *cols_out,
+                        int64_t col_stride, void *stream) {
+    return demod_launch(h, sfts, n_sky, delay, delay_rate, a, b, n_tmpl, freq, fdot, ref_time,
+                        dterms, ts_out, cols_out, col_stride, stream);
 }
 
 }  // extern "C"

@@ -5,7 +5,8 @@ FSTATQ_ATOMS_PER_DET)`` once per Doppler point on the host
 (``pyfstat/core.py:671-673``, ``:1359-1365``).  Here one call of
 ``tcw_demod_atoms`` produces the atoms of many (f, fdot) templates at one sky
 position from the SFT bins resident on the device, in the layout
-``tcw_batch_run`` reads; :func:`batch_from_sfts` chains the two so that a
+``tcw_batch_run`` reads (``tcw_demod_atoms_sky``: several sky positions in one
+launch, :func:`demod_atoms_sky`); :func:`batch_from_sfts` chains the two so that a
 search goes from SFTs to transient maps without a host round trip.
 
 The sum is the resampling-free demodulation F-statistic (see
@@ -159,6 +160,47 @@ def demod_atoms(sfts, freq, fdot, ref_time, dterms=8, stream=None):
         demod_atoms_into(sfts, f, fd, ref_time, ts_d, cols_d, dterms=dterms, stream=stream)
     out = DeviceAtoms(sfts, ts_d, cols_d, T)
     out._keep = (f, fd)  # inputs stay alive until the queued kernel has run
+    return out
+
+
+def demod_atoms_sky(sfts, delay, delay_rate, a, b, freq, fdot, ref_time, dterms=8,
+                    stream=None):
+    """Atoms of the templates (freq[i], fdot[i]) at each of n_sky sky
+    positions in one launch (``tcw_demod_atoms_sky``): ``delay``,
+    ``delay_rate``, ``a``, ``b`` are [n_sky, n_sft] (row s: sky point s's
+    per-SFT values, SFTs in ``sfts``' order; ``sfts``' own delay and antenna
+    patterns are not used).  Returns :class:`DeviceAtoms` of n_sky * T
+    templates, sky point s's template i being template s * T + i."""
+    torch = _torch()
+    n = sfts.n_sft
+    per_sky = []
+    for name, x, dt in (("delay", delay, np.float64), ("delay_rate", delay_rate, np.float64),
+                        ("a", a, np.float32), ("b", b, np.float32)):
+        x = np.ascontiguousarray(x, dtype=dt)
+        if x.ndim != 2 or x.shape[1] != n:
+            raise ValueError(f"{name} must be [n_sky, n_sft]")
+        per_sky.append(x)
+    n_sky = per_sky[0].shape[0]
+    if any(x.shape[0] != n_sky for x in per_sky):
+        raise ValueError("delay, delay_rate, a and b must have the same number of sky points")
+    f = np.ascontiguousarray(freq, dtype=np.float64).reshape(-1)
+    fd = np.ascontiguousarray(fdot, dtype=np.float64).reshape(-1)
+    if f.size != fd.size:
+        raise ValueError("freq and fdot must have the same length")
+    T = int(f.size)
+    dev = sfts.device
+    keep = [torch.from_numpy(x).to(dev) for x in per_sky + [f, fd]]
+    total = n_sky * T * n
+    ts_d = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    cols_d = torch.empty((NC, max(total, 1)), dtype=torch.float32, device=dev)
+    st = torch.cuda.current_stream(dev) if stream is None else stream
+    _hiplib.check(sfts.lib.tcw_demod_atoms_sky(
+        sfts.handle, ctypes.byref(sfts.desc), n_sky, *(x.data_ptr() for x in keep[:4]), T,
+        keep[4].data_ptr(), keep[5].data_ptr(), float(ref_time), int(dterms), ts_d.data_ptr(),
+        cols_d.data_ptr(), cols_d.shape[1], ctypes.c_void_p(st.cuda_stream)),
+        "tcw_demod_atoms_sky")
+    out = DeviceAtoms(sfts, ts_d, cols_d, n_sky * T)
+    out._keep = keep  # inputs stay alive until the queued kernel has run
     return out
 
 

@@ -11,6 +11,8 @@ are the float32 products of the float32 antenna patterns, bit-exact.  Maps from
 device atoms equal maps from the same atoms copied through the host
 (bit-exact: the same kernels on the same numbers)."""
 
+from types import SimpleNamespace
+
 import numpy as np
 import pytest
 
@@ -194,3 +196,42 @@ def test_validation(hip_ready):
     with pytest.raises(ValueError, match="band"):
         demod_atoms(sfts, [100.01], [0.0], sft.ref_time)
     sfts.desc.k_min = sft.k_min
+
+
+@pytest.mark.parametrize("T", [3, 40])
+def test_sky_points_in_one_launch(hip_ready, T):
+    """tcw_demod_atoms_sky: three sky points (their own per-SFT delay, rate
+    and antenna patterns) in one launch equal, bit for bit, three single-sky
+    launches with those arrays in the SFT set, and match the oracle."""
+    from pyfstat_amd.demod import SFTSet, demod_atoms, demod_atoms_sky
+
+    sft = synth.synthetic_sfts(n_det=2, n_sft=70, seed_index=76)
+    sfts = sft.sft_set()
+    n = sft.t_start.size
+    rng = np.random.default_rng(76 + T)
+    n_sky = 3
+    delay = np.stack([sft.delay * (1 - 0.3 * s) + 0.01 * s for s in range(n_sky)])
+    rate = np.stack([sft.delay_rate * (1 + 0.5 * s) for s in range(n_sky)])
+    a = np.stack([np.cos(0.7 * s) * sft.a - np.sin(0.7 * s) * sft.b for s in range(n_sky)])
+    b = np.stack([np.sin(0.7 * s) * sft.a + np.cos(0.7 * s) * sft.b for s in range(n_sky)])
+    freqs = rng.uniform(100.0, 100.05, T)
+    fdots = rng.uniform(-5e-9, 5e-9, T)
+    at = demod_atoms_sky(sfts, delay, rate, a, b, freqs, fdots, sft.ref_time)
+    assert at.T == n_sky * T
+    ts, cols = at.host()
+    for s in range(n_sky):
+        print(f"case: {T} templates, sky point {s}", flush=True)
+        one = SFTSet(sft.t_start, sft.bins, sft.k_min, sft.Tsft, delay[s], rate[s], a[s], b[s],
+                     sft.det_off)
+        ts1, cols1 = demod_atoms(one, freqs, fdots, sft.ref_time).host()
+        blk = slice(s * T * n, (s + 1) * T * n)
+        assert np.array_equal(ts[blk], ts1)
+        assert np.array_equal(cols[:, blk], cols1)
+        view = SimpleNamespace(t_start=sft.t_start, bins=sft.bins, k_min=sft.k_min,
+                               Tsft=sft.Tsft, delay=delay[s], delay_rate=rate[s], a=a[s],
+                               b=b[s], ref_time=sft.ref_time)
+        _check_atoms(ts1, cols1, view, freqs, fdots)
+    with pytest.raises(ValueError, match="n_sky, n_sft"):
+        demod_atoms_sky(sfts, delay[:, :-1], rate, a, b, freqs, fdots, sft.ref_time)
+    with pytest.raises(ValueError, match="same number of sky points"):
+        demod_atoms_sky(sfts, delay, rate[:2], a, b, freqs, fdots, sft.ref_time)
