@@ -645,17 +645,13 @@ struct Reducer {
 #pragma unroll
             for (int c = 0; c < NCOLS; c++) lm = fmaxf(lm, F[r][c]);
         if (lm > best || (lm == best && lm != -INFINITY)) {
-            int rr = 0, cc = 0;
-            bool found = false;
+            // the first cell (row-major) equal to lm: selects from the last
+            // cell to the first, no branches (a search with an early exit
+            // compiles to a chain of masked branches, one per cell)
+            int pos = 0;
 #pragma unroll
-            for (int r = 0; r < R; r++)
-#pragma unroll
-                for (int c = 0; c < NCOLS; c++)
-                    if (!found && F[r][c] == lm) {
-                        found = true;
-                        rr = r;
-                        cc = c;
-                    }
+            for (int q = R * NCOLS - 1; q >= 0; q--) pos = F[q / NCOLS][q % NCOLS] == lm ? q : pos;
+            const int rr = pos / NCOLS, cc = pos % NCOLS;
             // a lane's columns are evenly spaced (1 or 64 apart): arithmetic,
             // not a selection the compiler would turn into a private array
             int col = colidx[0];
