@@ -762,11 +762,20 @@ __device__ void finish_item(Reducer<BTSG, NCOLS> &red, int wave, int lane, const
 //   i(m)   = min(q0 + a*m, N-1)                      (rect.cu:21-31)
 //   j(D)   = min(max(q00 + a*D - 1, 0), N-1) + 1     (rect.cu:54-69), D = m+n,
 // and j >= i always, so the incremental sum of rect.cu:75-91 equals
-// P[j] - P[i].  The item's j(D) range is staged in LDS (component-major,
-// padded every 4 entries so the 4-column lane stride hits distinct banks);
-// each lane computes a 4x4 block whose 16 cells touch 7 diagonals, loading
-// 4 entries for the first row and one more per following row.
-__host__ __device__ __forceinline__ int lds_pad(int d) { return d + (d >> 2); }
+// P[j] - P[i].  The item's j(D) range is staged in LDS in groups of 4
+// diagonals, each group's 7 components x 4 diagonals contiguous plus one pad
+// double (LDS_GS doubles: a lane's 4-group stride of 58 words spreads its
+// 8-byte reads over all banks); each lane computes a 4x4 block whose 16
+// cells touch 7 diagonals -- its own group and 3 entries of the next, all at
+// constant offsets from one address -- loading 4 entries for the first row
+// and one more per following row.
+constexpr int LDS_GS = 4 * NC + 1;
+// entry (diagonal d, component k) of the staged range
+__host__ __device__ __forceinline__ int lds_at(int d, int k) {
+    return (d >> 2) * LDS_GS + k * 4 + (d & 3);
+}
+// doubles of LDS for nd diagonals
+__host__ __device__ __forceinline__ int lds_doubles(int nd) { return (nd + 3) / 4 * LDS_GS; }
 
 // Work-item order.  Blocks go to the 8 XCDs round-robin (block b on XCD
 // b % 8) and every XCD has its own L2, so XCD x is given a contiguous run of
@@ -807,7 +816,6 @@ void rect_fast_kernel(
     const int col0 = it.cb * TILE_COLS;
     const int N_t0 = (int)td.N_t0, N_tau = (int)td.N_tau;
     const int nd = RB + TILE_COLS - 1;
-    const int ndp = lds_pad(nd) + 1;
     const int D0 = row0 + col0;
     const double *__restrict__ P = prefix + td.prefix_off;
     const int Nm1 = (int)td.N - 1;
@@ -816,9 +824,8 @@ void rect_fast_kernel(
         int64_t q = (int64_t)td.q00 + (int64_t)td.a * (D0 + d) - 1;
         q = min(max(q, (int64_t)0), (int64_t)Nm1);
         const double *__restrict__ src = P + (q + 1) * NC;
-        double *dst = lds + lds_pad(d);
 #pragma unroll
-        for (int c = 0; c < NC; c++) dst[c * ndp] = src[c];
+        for (int c = 0; c < NC; c++) lds[lds_at(d, c)] = src[c];
     }
     __syncthreads();
 
@@ -841,13 +848,13 @@ void rect_fast_kernel(
         if (m0 >= N_t0) break;
         const int nvr = min(N_t0 - m0, 4);
         // lane's diagonals d = rl0 + 4*lane + x, x = 0..6; rl0 + 4*lane is a
-        // multiple of 4, so lds_pad(d) = lds_pad(rl0 + 4*lane) + x + x/4
-        const int lb = lds_pad(rl0 + lane * 4);
+        // multiple of 4, so lds_at(d, k) = lb + lds_at(x, k)
+        const double *__restrict__ lb = lds + lds_at(rl0 + lane * 4, 0);
         double w[4][NC];
 #pragma unroll
         for (int c = 0; c < 4; c++)
 #pragma unroll
-            for (int k = 0; k < NC; k++) w[c][k] = lds[k * ndp + lb + c];
+            for (int k = 0; k < NC; k++) w[c][k] = lb[lds_at(c, k)];
         float F[4][4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -858,7 +865,7 @@ void rect_fast_kernel(
                     for (int k = 0; k < NC; k++) w[c][k] = w[c + 1][k];
                 const int x = r + 3;
 #pragma unroll
-                for (int k = 0; k < NC; k++) w[3][k] = lds[k * ndp + lb + x + (x >> 2)];
+                for (int k = 0; k < NC; k++) w[3][k] = lb[lds_at(x, k)];
             }
             // window start: wave-uniform, read through the scalar cache
             const int i = min(td.q0 + td.a * (m0 + r), Nm1);
@@ -2458,7 +2465,7 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
             items_by_mode[mode].push_back(ItemDev{(int32_t)t, (int32_t)rbk, (int32_t)cbk, 0});
     if (mode == MODE_RECT_FAST) {
         const int nd = rb + TILE_COLS - 1;
-        const int bytes = (int)sizeof(double) * NC * (lds_pad(nd) + 1);
+        const int bytes = (int)sizeof(double) * lds_doubles(nd);
         p.max_lds_bytes = std::max(p.max_lds_bytes, bytes);
     }
     p.tmpl.push_back(td);
