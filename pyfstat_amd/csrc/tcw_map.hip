@@ -613,6 +613,10 @@ template <bool BTSG, int NCOLS = 4>
 struct Reducer {
     float best;
     int bidx;
+    // a lower bound of the wave's largest value so far (the wave maximum of
+    // best at the last search): a lane whose block maximum is below it holds
+    // no candidate for the item's first maximum and skips the search
+    float wbest;
     float Mw;
     float ML;
     float Scol[NCOLS];
@@ -620,6 +624,7 @@ struct Reducer {
     __device__ void init() {
         best = -INFINITY;
         bidx = 0x7fffffff;
+        wbest = -INFINITY;
         Mw = -INFINITY;
         ML = -INFINITY;
 #pragma unroll
@@ -644,7 +649,11 @@ struct Reducer {
         for (int r = 0; r < R; r++)
 #pragma unroll
             for (int c = 0; c < NCOLS; c++) lm = fmaxf(lm, F[r][c]);
-        if (lm > best || (lm == best && lm != -INFINITY)) {
+        // a block holding the item's largest value V always searches (V >=
+        // wbest, V >= best); a block below wbest has a larger value in
+        // another lane of the wave
+        const bool search = lm >= wbest && (lm > best || (lm == best && lm != -INFINITY));
+        if (search) {
             // the first cell (row-major) equal to lm: selects from the last
             // cell to the first, no branches (a search with an early exit
             // compiles to a chain of masked branches, one per cell)
@@ -658,6 +667,7 @@ struct Reducer {
             if constexpr (NCOLS > 1) col += cc * (colidx[1] - colidx[0]);
             argmax_merge(best, bidx, lm, (m0 + rr) * N_tau + col);
         }
+        if (__any(search)) wbest = wave_max(best);
         if (!BTSG) return;
         // exp(F - Mw) only has to stay finite, so the wave reference Mw is
         // raised (wave maximum, accumulators rescaled) only when some lane
