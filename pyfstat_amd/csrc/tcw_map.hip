@@ -651,8 +651,11 @@ struct Reducer {
             for (int c = 0; c < NCOLS; c++) lm = fmaxf(lm, F[r][c]);
         // a block holding the item's largest value V always searches (V >=
         // wbest, V >= best); a block below wbest has a larger value in
-        // another lane of the wave
-        const bool search = lm >= wbest && (lm > best || (lm == best && lm != -INFINITY));
+        // another lane of the wave.  Only for 16-cell blocks: for 4 cells the
+        // wave maximum that updates wbest costs more than the search
+        constexpr bool FILTER = R * NCOLS >= 16;
+        const bool search = (!FILTER || lm >= wbest) &&
+                            (lm > best || (lm == best && lm != -INFINITY));
         if (search) {
             // the first cell (row-major) equal to lm: selects from the last
             // cell to the first, no branches (a search with an early exit
@@ -667,7 +670,7 @@ struct Reducer {
             if constexpr (NCOLS > 1) col += cc * (colidx[1] - colidx[0]);
             argmax_merge(best, bidx, lm, (m0 + rr) * N_tau + col);
         }
-        if (__any(search)) wbest = wave_max(best);
+        if (FILTER && __any(search)) wbest = wave_max(best);
         if (!BTSG) return;
         // exp(F - Mw) only has to stay finite, so the wave reference Mw is
         // raised (wave maximum, accumulators rescaled) only when some lane
