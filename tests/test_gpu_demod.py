@@ -235,3 +235,6 @@ def test_sky_points_in_one_launch(hip_ready, T):
         demod_atoms_sky(sfts, delay[:, :-1], rate, a, b, freqs, fdots, sft.ref_time)
     with pytest.raises(ValueError, match="same number of sky points"):
         demod_atoms_sky(sfts, delay, rate[:2], a, b, freqs, fdots, sft.ref_time)
+    # the C ABI's own check: 1 .. 65535 sky points
+    with pytest.raises(ValueError, match="sky points must number"):
+        demod_atoms_sky(sfts, delay[:0], rate[:0], a[:0], b[:0], freqs, fdots, sft.ref_time)
