@@ -790,6 +790,10 @@ __host__ __device__ __forceinline__ int lds_at(int d, int k) {
 // doubles of LDS for nd diagonals
 __host__ __device__ __forceinline__ int lds_doubles(int nd) { return (nd + 3) / 4 * LDS_GS; }
 
+// Word 3 of a raw buffer resource: 32-bit data format, as gfx9 raw buffer
+// accesses require.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+
 // Work-item order.  Blocks go to the 8 XCDs round-robin (block b on XCD
 // b % 8) and every XCD has its own L2, so XCD x is given a contiguous run of
 // work items: the items of one template, which read the same prefix table,
@@ -854,6 +858,15 @@ void rect_fast_kernel(
 #pragma unroll
     for (int c = 0; c < 4; c++) colidx[c] = c0 + c;
     const bool cols_full = col0 + TILE_COLS <= N_tau;
+    // partial sub-tiles store through a buffer resource over their rows
+    // inside the map: the lane's column c at byte voff[c] of a row, or far
+    // outside the resource for a column past N_tau (the hardware drops the
+    // store; rows past the map lie beyond the resource as well), so no
+    // branch per cell
+    const uint32_t rowb = (uint32_t)N_tau * (uint32_t)sizeof(float);
+    uint32_t voff[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) voff[c] = c < nvc ? (uint32_t)(c0 + c) * 4u : 0x80000000u;
 
     for (int s = 0; s < RB / 16; s++) {
         const int rl0 = s * 16 + wave * 4;
@@ -906,13 +919,15 @@ void rect_fast_kernel(
                 for (int c = 0; c < 4; c++) Frow[c] = F[r][c];
             }
         } else {
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                Ft + (int64_t)m0 * N_tau, (short)0, (int)((uint32_t)nvr * rowb), BUF_RSRC_WORD3);
 #pragma unroll
             for (int r = 0; r < 4; r++)
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    const bool v = (r < nvr) && (c < nvc);
-                    if (v) Ft[(int64_t)(m0 + r) * N_tau + c0 + c] = F[r][c];
-                    F[r][c] = v ? F[r][c] : -INFINITY;
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(F[r][c]), rs,
+                                                          (int)voff[c], (int)(r * rowb), 0);
+                    F[r][c] = (r < nvr) && (c < nvc) ? F[r][c] : -INFINITY;
                 }
         }
         red.template add<4>(F, m0, colidx, N_tau, lane, rl0, sh_rowM, sh_rowS);
@@ -1626,9 +1641,6 @@ __global__ __launch_bounds__(SCAN_WG) void exp_scan_seq_kernel(const TmplDev *__
     for (int c = 0; c < NC; c++) expf_etab(etab, td, 0, c)[n] = E[c];
 }
 
-// Word 3 of a raw buffer resource: 32-bit data format, as gfx9 raw buffer
-// accesses require.
-constexpr int BUF_RSRC_WORD3 = 0x00020000;
 
 // Rows st .. sb of a chunk (one pass) in groups of 4 (the pass's rows are a
 // multiple of 4), reduced together.  Rows past the map (m > m_hi, only in a
@@ -2380,7 +2392,10 @@ int plan_template(tcw_plan &p, int64_t t, const tcw_window &win_in, uint32_t N,
         const bool regular = (w.dt0 % TAtom == 0) && (w.dtau % TAtom == 0) && (w.dt0 == w.dtau);
         const uint64_t x_lo = (uint64_t)w.t0 + half;  // must not be below t_min
         const uint64_t x_hi = (uint64_t)w.t0 + w.t0Band + w.tau + w.tauBand + half;
-        if (!(p.flags & TCW_FLAG_GENERIC) && regular && x_lo >= t_min && x_hi < (1ull << 32)) {
+        // (rows of at most 2^20 cells: partial sub-tiles store by 32-bit byte
+        // offsets within their four rows)
+        if (!(p.flags & TCW_FLAG_GENERIC) && regular && x_lo >= t_min && x_hi < (1ull << 32) &&
+            N_tau <= (int64_t)1 << 20) {
             mode = MODE_RECT_FAST;
             td.a = (int32_t)(w.dt0 / TAtom);
             td.q0 = (int32_t)(((uint64_t)w.t0 - t_min + half) / TAtom);
